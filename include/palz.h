@@ -1,0 +1,161 @@
+/*
+ * palz.h — C ABI of the MI355X (gfx950) caustic hot path.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns an
+ * int status (PALZ_OK == 0) and is stream ordered on the `stream` argument
+ * (a hipStream_t passed as void*; NULL = the legacy default stream).  Device
+ * pointers are caller-owned unless stated otherwise.  Handles are not
+ * thread-safe; use one handle per host thread.  palz_last_error() returns a
+ * thread-local message for the last non-zero status.
+ *
+ * The POD records below reproduce the reference's in-memory layouts byte for
+ * byte so that buffers can be exchanged with code that uses the reference
+ * structs (sizes are static_asserted in the implementation).
+ */
+#ifndef PALZ_H
+#define PALZ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    PALZ_OK = 0,
+    PALZ_ERR_INVALID_ARGUMENT = 1,
+    PALZ_ERR_HIP = 2,            /* a HIP runtime call failed                    */
+    PALZ_ERR_OUT_OF_MEMORY = 3,
+    PALZ_ERR_CAPACITY = 4,       /* an output or scratch capacity was exceeded   */
+    PALZ_ERR_TRAVERSAL_STACK = 5 /* a BVH traversal needed a deeper stack        */
+};
+
+/* ---- POD records (layouts of the reference) ---------------------------- */
+
+/* float3: 12 B, no padding (helper_math.h float3). */
+typedef struct { float x, y, z; } palz_float3;
+
+/* minimum_lbvh::Triangle (minimum_lbvh.h:207-210): 36 B. */
+typedef struct { palz_float3 vs[3]; } palz_triangle;
+
+/* minimum_lbvh::AABB (minimum_lbvh.h:220-261): 24 B. */
+typedef struct { palz_float3 lower, upper; } palz_aabb;
+
+/* minimum_lbvh::NodeIndex (minimum_lbvh.h:263-274): bits [0,31) index, bit 31
+ * leaf flag.  The invalid node is 0x7FFFFFFF (index 0x7FFFFFFF, not a leaf). */
+typedef uint32_t palz_node_index;
+#define PALZ_NODE_INVALID 0x7FFFFFFFu
+#define PALZ_NODE_LEAF_BIT 0x80000000u
+
+/* minimum_lbvh::InternalNode (minimum_lbvh.h:285-291): 64 B.
+ * `context` holds the escape link after a build: the upper bound of the
+ * node's leaf range when that is < n-1, else 0xFFFFFFFF. */
+typedef struct {
+    palz_node_index children[2];
+    palz_aabb aabbs[2];
+    palz_node_index parent;
+    uint32_t context;
+} palz_internal_node;
+
+/* Material (pk.h:67-72) and TriangleAttrib (pk.h:74-78): 40 B. */
+enum { PALZ_MATERIAL_DIFFUSE = 0, PALZ_MATERIAL_MIRROR = 1, PALZ_MATERIAL_DIELECTRIC = 2 };
+typedef struct {
+    int32_t material;
+    palz_float3 shading_normals[3];
+} palz_triangle_attrib;
+
+/* BUILD_OPTION (minimum_lbvh.h:42-46). USE_NORMAL follows the GPU packing of
+ * minimum_lbvh.cu:85; CPU_PARALLEL has no meaning on the GPU and is ignored. */
+enum { PALZ_BUILD_OPTION_USE_NORMAL = 1, PALZ_BUILD_OPTION_CPU_PARALLEL = 2 };
+
+/* ---- library ------------------------------------------------------------ */
+const char* palz_last_error(void);
+int palz_version(void);
+
+/* ---- device memory, streams, events ---------------------------------------
+ * Thin wrappers over the HIP runtime so that hosts without a HIP binding
+ * (ctypes, cgo, JNI) can drive the library.  They stand in for the
+ * reference's TypedBuffer / oroMalloc / DeviceStopwatch plumbing
+ * (typedbuffer.h:19-146, pk.h:1192-1223). */
+int palz_device_count(int* count);
+int palz_set_device(int device);
+int palz_malloc(void** d_ptr, size_t bytes);
+int palz_free(void* d_ptr);
+int palz_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes, void* stream);
+int palz_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes, void* stream);
+int palz_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes, void* stream);
+int palz_memset(void* d_dst, int value, size_t bytes, void* stream);
+int palz_stream_create(void** stream);
+int palz_stream_destroy(void* stream);
+int palz_stream_sync(void* stream);
+int palz_device_sync(void);
+int palz_event_create(void** event);
+int palz_event_destroy(void* event);
+int palz_event_record(void* event, void* stream);
+int palz_event_elapsed_ms(float* ms, void* start, void* stop);
+/* A plain streaming copy kernel (16-byte aligned, size a multiple of 16): the
+ * yardstick for achievable HBM bandwidth and for HBM counter calibration. */
+int palz_copy_bytes(void* d_dst, const void* d_src, size_t bytes, void* stream);
+
+/* ---- onesweep radix sort -------------------------------------------------
+ * Replaces tinyhiponesweep::OnesweepSort (tinyhiponesweep.h:572-668).
+ * The sorter owns its scratch (digit histograms, look-back table, tile
+ * tickets) and grows it on demand. */
+typedef struct palz_sorter palz_sorter;
+int palz_sorter_create(palz_sorter** out);
+int palz_sorter_destroy(palz_sorter* s);
+/* Pre-size the look-back table for n keys.  Sorting more keys than reserved
+ * grows it with hipMalloc, so reserve before capturing a sort in a hipGraph.
+ * (The reference sizes its table once, tinyhiponesweep.h:725-732.) */
+int palz_sorter_reserve(palz_sorter* s, uint32_t n);
+
+/* Stable LSD sort of 64-bit words on bit range [start_bit, end_bit), 8-bit
+ * digits.  Replaces OnesweepSort::sort(const KeyValueSoA64&, ..., n, startBit,
+ * endBit, stream) with `value == nullptr` (tinyhiponesweep.h:596-599), which is
+ * how both reference call sites use it (minimum_lbvh.h:848, pk.h:1526).  The
+ * result is written back to `words`; `tmp` (n words) is clobbered.  If
+ * `values` is not NULL the u32 payload moves with its word (`tmp_values`
+ * must then hold n entries).  n must be < 2^30. */
+int palz_sort_u64(palz_sorter* s, uint64_t* words, uint64_t* tmp, uint32_t* values,
+                  uint32_t* tmp_values, uint32_t n, int start_bit, int end_bit, void* stream);
+
+/* ---- LBVH builder ---------------------------------------------------------
+ * Replaces minimum_lbvh::BVHGPUBuilder (minimum_lbvh.h:741-900).  The builder
+ * owns the node array (n-1 records) and the root index, as the reference does
+ * (m_internals, m_rootNode). */
+typedef struct palz_lbvh palz_lbvh;
+int palz_lbvh_create(palz_lbvh** out);
+int palz_lbvh_destroy(palz_lbvh* b);
+
+/* BVHGPUBuilder::build(triangles, nTriangles, buildOption, sorter, stream)
+ * (minimum_lbvh.h:798).  morton_bits = 32 gives the reference key (top 32 bits
+ * of a 63-bit Morton code, minimum_lbvh.cu:89); 30 gives a 10-bit-per-axis
+ * Morton key.  d_tris is a device array of n triangles. */
+int palz_lbvh_build(palz_lbvh* b, const palz_triangle* d_tris, int n, uint32_t build_option,
+                    int morton_bits, palz_sorter* sorter, void* stream);
+
+/* Device pointers to the result (valid until the next build or destroy).
+ * *d_nodes has n-1 records, *d_root one palz_node_index, *d_scene_aabb one
+ * palz_aabb.  Any out-pointer may be NULL. */
+int palz_lbvh_result(palz_lbvh* b, palz_internal_node** d_nodes, palz_node_index** d_root,
+                     palz_aabb** d_scene_aabb, int* n_triangles);
+
+/* Device pointer to the sorted {key, index} words of the last build (n words,
+ * key in bits [0,32), triangle index in bits [32,64)) and the n-1 deltas;
+ * exposed for parity checks. */
+int palz_lbvh_sorted(palz_lbvh* b, uint64_t** d_words, uint8_t** d_deltas);
+
+/* Stage timing (replaces the commented-out DeviceStopwatch of
+ * minimum_lbvh.h:800-801, :884-885).  When enabled, each build records HIP
+ * events on its stream after every stage; palz_lbvh_timings waits for the
+ * last one and returns up to 8 durations in ms: [0] scene AABB, [1] Morton
+ * keys + digit counts, [2..5] the four sort passes, [6] deltas, [7] build. */
+int palz_lbvh_set_timing(palz_lbvh* b, int enable);
+int palz_lbvh_timings(palz_lbvh* b, float* ms, int max_stages, int* n_stages);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PALZ_H */

@@ -1,0 +1,34 @@
+"""Build recipe of the CPU oracle (test infrastructure): oracle/liboracle.so.
+
+    python oracle/build.py
+
+g++ -O3 with -ffp-contract=off and no fast-math, so the FP32 restatement does
+the same IEEE operations in the same order as the gfx950 kernels.  The target
+is x86-64-v3 (AVX2/FMA hosts) rather than -march=native so that the library
+built here also loads on the GPU host.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SOURCES = ["lbvh.cpp"]
+OUT = os.path.join(HERE, "liboracle.so")
+FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-march=x86-64-v3", "-ffp-contract=off",
+         "-fno-fast-math", "-Wall", "-Wextra", "-Wno-unused-parameter", "-pthread"]
+
+
+def build(verbose: bool = False) -> str:
+    srcs = [os.path.join(HERE, s) for s in SOURCES]
+    cmd = ["g++", *FLAGS, "-o", OUT + ".tmp", *srcs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(OUT + ".tmp", OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

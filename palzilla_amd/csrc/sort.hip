@@ -1,0 +1,347 @@
+// Onesweep LSD radix sort for gfx950 (wave64).
+//
+// Semantics follow tinyhiponesweep::OnesweepSort::sort (tinyhiponesweep.h:
+// 592-668): a stable LSD sort with 8-bit digits over [startBit, endBit) of
+// 64-bit words, optional u32 payload.  The design is MI355X-native:
+//   * one digit count pass reads the words once for every digit position
+//     (LDS histograms, one global atomic per non-empty bin and block);
+//   * each digit pass is ONE kernel: a 4096-key tile per 256-lane block, ranked
+//     by a wave64 multisplit (8 64-bit ballots per 64 keys, leader-lane LDS
+//     counters), a decoupled look-back across tiles in a table of 64-bit
+//     (epoch, flag, count) words, LDS staging and a digit-run coalesced scatter;
+//   * tiles are handed out by a ticket counter, so a block only ever waits on
+//     tiles owned by blocks that started before it;
+//   * look-back words carry an epoch that a device-side counter bumps at the
+//     start of every sort, so the table is never cleared and the sort can be
+//     captured in a hipGraph and replayed.
+#include "sort.hpp"
+
+#include <algorithm>
+
+namespace palz {
+
+namespace {
+
+constexpr uint32_t kFlagAggregate = 1u;
+constexpr uint32_t kFlagInclusive = 2u;
+
+__device__ __forceinline__ uint64_t lb_pack(uint32_t ep, uint32_t flag, uint32_t count) {
+    return ((uint64_t)((ep << 2) | flag) << 32) | (uint64_t)count;
+}
+
+__device__ __forceinline__ uint32_t digit_of(uint64_t w, int shift, uint32_t mask) {
+    return (uint32_t)(w >> shift) & mask;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_tot) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_tot[wave] = x;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int w = 0; w < wave; ++w) add += s_tot[w];
+    __syncthreads();
+    return x - v + add;
+}
+
+__global__ void __launch_bounds__(256) k_sort_prepare(uint32_t* hist, uint32_t* tickets, uint32_t* epoch,
+                                                      palz_aabb* aabb) {
+    for (int i = threadIdx.x; i < kSortMaxPasses * 256; i += blockDim.x) hist[i] = 0u;
+    if (threadIdx.x < kSortMaxPasses) tickets[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) {
+        epoch[0] = epoch[0] + 1u;
+        if (aabb) {
+            aabb->lower = {kFltMax, kFltMax, kFltMax};
+            aabb->upper = {-kFltMax, -kFltMax, -kFltMax};
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sort_count(const uint64_t* __restrict__ words, uint32_t n,
+                                                    int start_bit, int end_bit, int npass,
+                                                    uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_h[kSortMaxPasses * 256];
+    for (int i = threadIdx.x; i < npass * 256; i += 256) s_h[i] = 0u;
+    __syncthreads();
+    const uint32_t stride = gridDim.x * 256u;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += stride) {
+        const uint64_t w = words[i];
+        for (int p = 0; p < npass; ++p) {
+            const int shift = start_bit + 8 * p;
+            const int bits = min(8, end_bit - shift);
+            atomicAdd(&s_h[p * 256 + digit_of(w, shift, (1u << bits) - 1u)], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < npass * 256; i += 256) {
+        const uint32_t c = s_h[i];
+        if (c) atomicAdd(&hist[i], c);
+    }
+}
+
+template <bool kValues>
+__global__ void __launch_bounds__(kSortThreads)
+    k_sort_pass(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
+                const uint32_t* __restrict__ vsrc, uint32_t* __restrict__ vdst, uint32_t n,
+                int shift, uint32_t mask, int pass, const uint32_t* __restrict__ hist,
+                uint64_t* __restrict__ lookback, uint32_t* __restrict__ ticket,
+                const uint32_t* __restrict__ epoch) {
+    __shared__ uint32_t s_wave_hist[kSortWaves][256];
+    __shared__ int32_t s_local_start[256];
+    __shared__ int32_t s_scatter_base[256];
+    __shared__ uint32_t s_tot[kSortWaves];
+    __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kValues ? kSortTile : 1];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+
+    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) s_wave_hist[w][tid] = 0u;
+    __syncthreads();
+
+    const uint32_t tile = s_tile;
+    const uint32_t ep = epoch[0] * 8u + (uint32_t)pass;
+    const uint32_t tile_base = tile * (uint32_t)kSortTile;
+    const uint32_t wave_base = tile_base + (uint32_t)wave * (64u * kSortItems);
+
+    uint64_t key[kSortItems];
+    uint32_t val[kValues ? kSortItems : 1];
+    uint32_t rank[kSortItems];
+
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const uint32_t idx = wave_base + (uint32_t)j * 64u + (uint32_t)lane;
+        key[j] = idx < n ? src[idx] : 0ull;
+        if constexpr (kValues) val[j] = idx < n ? vsrc[idx] : 0u;
+    }
+
+    // Wave64 multisplit: rank each key among equal digits, in key order.
+    const uint64_t lt_mask = __lanemask_lt();
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const uint32_t idx = wave_base + (uint32_t)j * 64u + (uint32_t)lane;
+        const bool valid = idx < n;
+        const uint32_t d = digit_of(key[j], shift, mask);
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            m &= bit ? bal : ~bal;
+        }
+        const int leader = valid ? (__ffsll((unsigned long long)m) - 1) : lane;
+        uint32_t old = 0u;
+        if (valid && lane == leader) {
+            old = s_wave_hist[wave][d];
+            s_wave_hist[wave][d] = old + (uint32_t)__popcll(m);
+        }
+        old = (uint32_t)__shfl((int)old, leader, 64);
+        rank[j] = old + (uint32_t)__popcll(m & lt_mask);
+    }
+    __syncthreads();
+
+    // Per digit (one per thread): exclusive prefix over waves, tile count.
+    const uint32_t d = (uint32_t)tid;
+    uint32_t count = 0u;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) {
+        const uint32_t c = s_wave_hist[w][d];
+        s_wave_hist[w][d] = count;
+        count += c;
+    }
+
+    // Decoupled look-back over previous tiles for this digit.
+    uint64_t* my = lookback + (size_t)tile * 256u + d;
+    uint32_t excl = 0u;
+    if (tile == 0) {
+        __hip_atomic_store(my, lb_pack(ep, kFlagInclusive, count), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(my, lb_pack(ep, kFlagAggregate, count), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        int p = (int)tile - 1;
+        while (p >= 0) {
+            const uint64_t e = __hip_atomic_load(lookback + (size_t)p * 256u + d, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t hi = (uint32_t)(e >> 32);
+            if ((hi >> 2) != ep) {
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += (uint32_t)e;
+            if ((hi & 3u) == kFlagInclusive) break;
+            --p;
+        }
+        __hip_atomic_store(my, lb_pack(ep, kFlagInclusive, excl + count), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+
+    const uint32_t global_start = block_excl_scan256(hist[d], s_tot);
+    const uint32_t local_start = block_excl_scan256(count, s_tot);
+    s_local_start[d] = (int32_t)local_start;
+    s_scatter_base[d] = (int32_t)(global_start + excl) - (int32_t)local_start;
+    __syncthreads();
+
+    // Stage the tile in LDS in digit order.
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const uint32_t idx = wave_base + (uint32_t)j * 64u + (uint32_t)lane;
+        if (idx < n) {
+            const uint32_t dj = digit_of(key[j], shift, mask);
+            const uint32_t pos = (uint32_t)s_local_start[dj] + s_wave_hist[wave][dj] + rank[j];
+            s_keys[pos] = key[j];
+            if constexpr (kValues) s_vals[pos] = val[j];
+        }
+    }
+    __syncthreads();
+
+    // Digit runs are contiguous in LDS and in the output: coalesced scatter.
+    const uint32_t tile_n = min((uint32_t)kSortTile, n - tile_base);
+    for (uint32_t i = (uint32_t)tid; i < tile_n; i += kSortThreads) {
+        const uint64_t k = s_keys[i];
+        const uint32_t di = digit_of(k, shift, mask);
+        const uint32_t g = (uint32_t)(s_scatter_base[di] + (int32_t)i);
+        dst[g] = k;
+        if constexpr (kValues) vdst[g] = s_vals[i];
+    }
+}
+
+}  // namespace
+
+void launch_sort_prepare(const SortScratch& s, palz_aabb* aabb, hipStream_t stream) {
+    k_sort_prepare<<<1, 256, 0, stream>>>(s.hist, s.tickets, s.epoch, aabb);
+}
+
+void launch_sort_count(const SortScratch& s, const uint64_t* words, uint32_t n, int start_bit,
+                       int end_bit, hipStream_t stream) {
+    const int npass = (int)div_up((uint32_t)(end_bit - start_bit), 8u);
+    const uint32_t blocks = std::max(1u, std::min(div_up(n, 256u * 16u), 2048u));
+    k_sort_count<<<blocks, 256, 0, stream>>>(words, n, start_bit, end_bit, npass, s.hist);
+}
+
+int launch_sort_passes(const SortScratch& s, uint64_t* words, uint64_t* tmp, uint32_t* values,
+                       uint32_t* tmp_values, uint32_t n, int start_bit, int end_bit,
+                       hipStream_t stream, hipEvent_t* pass_events) {
+    const int npass = (int)div_up((uint32_t)(end_bit - start_bit), 8u);
+    const uint32_t tiles = div_up(n, (uint32_t)kSortTile);
+    uint64_t* src = words;
+    uint64_t* dst = tmp;
+    uint32_t* vsrc = values;
+    uint32_t* vdst = tmp_values;
+    for (int p = 0; p < npass; ++p) {
+        const int shift = start_bit + 8 * p;
+        const int bits = std::min(8, end_bit - shift);
+        const uint32_t mask = (1u << bits) - 1u;
+        if (values) {
+            k_sort_pass<true><<<tiles, kSortThreads, 0, stream>>>(
+                src, dst, vsrc, vdst, n, shift, mask, p, s.hist + p * 256, s.lookback,
+                s.tickets + p, s.epoch);
+        } else {
+            k_sort_pass<false><<<tiles, kSortThreads, 0, stream>>>(
+                src, dst, nullptr, nullptr, n, shift, mask, p, s.hist + p * 256, s.lookback,
+                s.tickets + p, s.epoch);
+        }
+        if (pass_events) (void)hipEventRecord(pass_events[p], stream);
+        std::swap(src, dst);
+        std::swap(vsrc, vdst);
+    }
+    return npass;
+}
+
+}  // namespace palz
+
+int palz_sorter::reserve(uint32_t n) {
+    using namespace palz;
+    if (!scratch.hist) {
+        PALZ_HIP(hipMalloc(&scratch.hist, sizeof(uint32_t) * kSortMaxPasses * 256));
+        PALZ_HIP(hipMalloc(&scratch.tickets, sizeof(uint32_t) * kSortMaxPasses));
+        PALZ_HIP(hipMalloc(&scratch.epoch, sizeof(uint32_t)));
+        PALZ_HIP(hipMemset(scratch.epoch, 0, sizeof(uint32_t)));
+    }
+    const uint32_t tiles = std::max(1u, div_up(n, (uint32_t)kSortTile));
+    if (tiles > scratch.lookback_tiles) {
+        // Grow geometrically; fresh words are zero (epoch 0 never matches).
+        const uint32_t want = std::max(tiles, scratch.lookback_tiles * 2u);
+        uint64_t* table = nullptr;
+        PALZ_HIP(hipMalloc(&table, sizeof(uint64_t) * 256u * want));
+        PALZ_HIP(hipMemset(table, 0, sizeof(uint64_t) * 256u * want));
+        PALZ_HIP(hipDeviceSynchronize());
+        if (scratch.lookback) PALZ_HIP(hipFree(scratch.lookback));
+        scratch.lookback = table;
+        scratch.lookback_tiles = want;
+    }
+    return PALZ_OK;
+}
+
+palz_sorter::~palz_sorter() {
+    if (scratch.hist) (void)hipFree(scratch.hist);
+    if (scratch.tickets) (void)hipFree(scratch.tickets);
+    if (scratch.epoch) (void)hipFree(scratch.epoch);
+    if (scratch.lookback) (void)hipFree(scratch.lookback);
+}
+
+extern "C" {
+
+int palz_sorter_create(palz_sorter** out) {
+    if (!out) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_sorter_create: out is NULL");
+    palz_sorter* s = new palz_sorter();
+    const int rc = s->reserve(1u << 20);
+    if (rc != PALZ_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return PALZ_OK;
+}
+
+int palz_sorter_destroy(palz_sorter* s) {
+    delete s;
+    return PALZ_OK;
+}
+
+int palz_sorter_reserve(palz_sorter* s, uint32_t n) {
+    if (!s) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_sorter_reserve: sorter is NULL");
+    if (n > palz::kSortMaxN) return palz::fail(PALZ_ERR_CAPACITY, "palz_sorter_reserve: n >= 2^30");
+    return s->reserve(n);
+}
+
+int palz_sort_u64(palz_sorter* s, uint64_t* words, uint64_t* tmp, uint32_t* values,
+                  uint32_t* tmp_values, uint32_t n, int start_bit, int end_bit, void* stream) {
+    using namespace palz;
+    if (!s) return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_sort_u64: sorter is NULL");
+    if (start_bit < 0 || end_bit > 64 || start_bit >= end_bit)
+        return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_sort_u64: need 0 <= start_bit < end_bit <= 64");
+    if (n > kSortMaxN) return fail(PALZ_ERR_CAPACITY, "palz_sort_u64: n must be < 2^30");
+    if (n == 0) return PALZ_OK;
+    if (!words || !tmp) return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_sort_u64: words/tmp is NULL");
+    if ((values == nullptr) != (tmp_values == nullptr))
+        return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_sort_u64: values and tmp_values go together");
+    const int rc = s->reserve(n);
+    if (rc != PALZ_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    launch_sort_prepare(s->scratch, nullptr, st);
+    launch_sort_count(s->scratch, words, n, start_bit, end_bit, st);
+    const int npass = launch_sort_passes(s->scratch, words, tmp, values, tmp_values, n, start_bit,
+                                         end_bit, st);
+    PALZ_LAUNCH_CHECK();
+    if (npass & 1) {
+        PALZ_HIP(hipMemcpyAsync(words, tmp, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, st));
+        if (values)
+            PALZ_HIP(hipMemcpyAsync(values, tmp_values, sizeof(uint32_t) * n,
+                                    hipMemcpyDeviceToDevice, st));
+    }
+    return PALZ_OK;
+}
+
+}  // extern "C"

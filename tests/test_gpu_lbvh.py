@@ -1,0 +1,134 @@
+"""GPU parity of the onesweep sort and the LBVH build (bit-exact vs the CPU
+oracle and numpy's stable argsort)."""
+import numpy as np
+import pytest
+
+import oracle
+from palzilla_amd import scenes
+from palzilla_amd import types as T
+from palzilla_amd.device import DeviceBuffer
+from palzilla_amd.lbvh import BVHGPUBuilder, OnesweepSort
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_sort(sorter, words, start, end, values=None):
+    n = words.size
+    dw = DeviceBuffer.from_array(words)
+    dt = DeviceBuffer(8 * max(n, 1))
+    dv = dtv = None
+    if values is not None:
+        dv = DeviceBuffer.from_array(values)
+        dtv = DeviceBuffer(4 * max(n, 1))
+    sorter.sort(dw.ptr, dt.ptr, n, start, end, values=dv.ptr if dv else 0,
+                tmp_values=dtv.ptr if dtv else 0)
+    out = dw.download(np.uint64, n)
+    if values is None:
+        return out
+    return out, dv.download(np.uint32, n)
+
+
+@pytest.mark.parametrize("n", [1, 63, 4095, 4096, 4097, 65539, 1 << 20])
+@pytest.mark.parametrize("start,end", [(0, 32), (0, 64), (3, 29)])
+def test_sort_matches_stable_argsort(sorter, n, start, end):
+    print(f"sort n={n} bits=[{start},{end})", flush=True)
+    rng = np.random.default_rng(n + start * 7 + end)
+    w = rng.integers(0, 2 ** 64 - 1, size=n, dtype=np.uint64)
+    if n > 10:
+        w[:: 5] = w[1]  # long runs of equal keys test stability
+    v = np.arange(n, dtype=np.uint32)[::-1].copy()
+    width = end - start
+    mask = np.uint64(2 ** 64 - 1) if width == 64 else np.uint64((1 << width) - 1)
+    order = np.argsort((w >> np.uint64(start)) & mask, kind="stable")
+    got_w, got_v = _gpu_sort(sorter, w, start, end, values=v)
+    assert np.array_equal(got_w, w[order])
+    assert np.array_equal(got_v, v[order])
+    got_only = _gpu_sort(sorter, w, start, end)
+    assert np.array_equal(got_only, w[order])
+
+
+def test_sort_skewed_digits(sorter):
+    # all keys share the high digits: one bin holds the whole tile in late passes
+    n = 300_000
+    rng = np.random.default_rng(5)
+    w = rng.integers(0, 256, size=n, dtype=np.uint64) | np.uint64(0xABCD0000)
+    order = np.argsort(w & np.uint64(0xFFFFFFFF), kind="stable")
+    assert np.array_equal(_gpu_sort(sorter, w, 0, 32), w[order])
+
+
+def test_sort_repeat_calls_reuse_lookback(sorter):
+    rng = np.random.default_rng(9)
+    for i in range(6):
+        n = int(rng.integers(1, 200_000))
+        w = rng.integers(0, 2 ** 40, size=n, dtype=np.uint64)
+        order = np.argsort(w & np.uint64(0xFFFFFFFFFF), kind="stable")
+        assert np.array_equal(_gpu_sort(sorter, w, 0, 40), w[order])
+
+
+def _gpu_lbvh(sorter, tris, bits, option=0):
+    dt = DeviceBuffer.from_array(tris)
+    b = BVHGPUBuilder()
+    b.build(dt.ptr, tris.shape[0], option, sorter, morton_bits=bits)
+    return b
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 256, 257, 4097, 100_003, 1 << 20])
+@pytest.mark.parametrize("bits", [30, 32])
+def test_lbvh_bit_exact(sorter, n, bits):
+    print(f"lbvh n={n} bits={bits}", flush=True)
+    tris = scenes.triangle_soup(n, seed=n * 3 + bits)
+    if n > 8:
+        tris[n // 2: n // 2 + 5] = tris[n // 3]
+    nodes, root, words, deltas, box = oracle.lbvh(tris, bits=bits)
+    b = _gpu_lbvh(sorter, tris, bits)
+    g_box = b.download_scene_aabb()
+    assert g_box.tobytes() == box.tobytes()
+    g_words, g_deltas = b.download_sorted()
+    assert np.array_equal(g_words, words)
+    assert np.array_equal(g_deltas, deltas)
+    assert b.download_root() == root
+    g_nodes = b.download_nodes()
+    assert g_nodes.tobytes() == nodes.tobytes()
+
+
+def test_lbvh_use_normal_option(sorter):
+    tris = scenes.triangle_soup(5000, seed=77)
+    nodes, root, words, deltas, box = oracle.lbvh(tris, option=T.BUILD_OPTION_USE_NORMAL, bits=32)
+    b = _gpu_lbvh(sorter, tris, 32, option=T.BUILD_OPTION_USE_NORMAL)
+    g_words, _ = b.download_sorted()
+    assert np.array_equal(g_words, words)
+    assert b.download_nodes().tobytes() == nodes.tobytes()
+
+
+def test_lbvh_rebuild_and_unaligned_input(sorter):
+    # a second build of a different size through the same builder, from a
+    # triangle array that is not 16-byte aligned (staging falls back to dwords)
+    b = BVHGPUBuilder()
+    for n, off in ((5000, 0), (777, 1)):
+        tris = scenes.triangle_soup(n, seed=n)
+        buf = np.zeros((n + 1) * 9, dtype=np.float32)
+        buf[off * 9: off * 9 + n * 9] = tris.reshape(-1)
+        dt = DeviceBuffer.from_array(buf)
+        b.build(dt.ptr + off * 36, n, 0, sorter, morton_bits=32)
+        nodes, root, *_ = oracle.lbvh(tris, bits=32)
+        assert b.download_root() == root
+        assert b.download_nodes().tobytes() == nodes.tobytes()
+
+
+def test_lbvh_heightfield_grid(sorter):
+    # structured input: a regular triangulated grid (many equal-delta runs)
+    m = 64
+    x, z = np.meshgrid(np.linspace(-1, 1, m + 1), np.linspace(-1, 1, m + 1), indexing="ij")
+    y = 0.5 + 0.02 * np.sin(3 * x) * np.cos(5 * z)
+    P = np.stack([x, y, z], axis=-1)
+    tris = []
+    for i in range(m):
+        for j in range(m):
+            a, b_, c, d = P[i, j], P[i + 1, j], P[i + 1, j + 1], P[i, j + 1]
+            tris += [np.concatenate([a, b_, c]), np.concatenate([a, c, d])]
+    tris = np.array(tris, dtype=np.float32)
+    for bits in (30, 32):
+        nodes, root, *_ = oracle.lbvh(tris, bits=bits)
+        b = _gpu_lbvh(sorter, tris, bits)
+        assert b.download_root() == root
+        assert b.download_nodes().tobytes() == nodes.tobytes()
