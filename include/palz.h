@@ -155,6 +155,137 @@ int palz_lbvh_sorted(palz_lbvh* b, uint64_t** d_words, uint8_t** d_deltas);
 int palz_lbvh_set_timing(palz_lbvh* b, int enable);
 int palz_lbvh_timings(palz_lbvh* b, float* ms, int max_stages, int* n_stages);
 
+/* ---- scenes, photons, tuples, paths ------------------------------------ */
+
+/* FirstDiffuse (pk.h:80-86): a receiver point.  R == (0,0,0) marks it
+ * invalid (main_gpu.cu:385); lambda is ignored by the solvers here, which
+ * take an explicit wavelength list. */
+typedef struct {
+    palz_float3 p;
+    palz_float3 ng;
+    palz_float3 R;
+    float lambda;
+} palz_first_diffuse;
+
+/* Device pointers of one scene (triangles, attributes and its BVH). */
+typedef struct {
+    const palz_triangle* tris;
+    const palz_triangle_attrib* attribs;
+    const palz_internal_node* nodes;
+    const palz_node_index* root;
+    uint32_t n_tris;
+    uint32_t pad_;
+} palz_scene_view;
+
+/* photonTrace arguments (main_gpu.cu:56, pk.h:1441-1459). */
+typedef struct {
+    palz_float3 p_light;
+    float eta_min;               /* n(390 nm) in the reference: lerp end points */
+    float eta_max;
+    int32_t iteration;
+    float min_throughput;        /* m_minThroughput = 0.05 (pk.h:1687) */
+    uint32_t photons_per_triangle;  /* the reference launches 256 (pk.h:1456-1457) */
+} palz_photon_params;
+
+/* One photon's specular tuple: PathCache::TrianglePath (pk.h:234-240) plus the
+ * photon id and landing point.  Records are dense by photon id =
+ * triangle * photons_per_triangle + j; event == 0 marks "no tuple". */
+typedef struct {
+    uint32_t photon;
+    uint32_t event;       /* EventDescriptor::asU32 (pk.h:156-158) */
+    int32_t tris[4];
+    float params[8];
+    palz_float3 p_final;
+    float throughput;
+} palz_photon_record;
+
+/* One (receiver point, tuple) work item: SpecularPath (pk.h:1102-1106) with
+ * the cached tuple and its initial barycentrics inlined. */
+typedef struct {
+    uint32_t scene;       /* index into a scene table (0 for a single scene) */
+    uint32_t pixel;       /* receiver point index */
+    uint32_t event;       /* EventDescriptor::asU32; size = event >> 24 */
+    uint32_t photon;      /* id of the photon that seeded the tuple */
+    int32_t tris[4];
+    float params[8];
+} palz_path;
+
+/* photonTrace (main_gpu.cu:56-211) over every triangle of the scene,
+ * `photons_per_triangle` photons each.  d_out must hold n_tris *
+ * photons_per_triangle records.  The traversal stack lives in LDS; a ray that
+ * would overflow it sets *d_status (device, may be NULL) to
+ * PALZ_ERR_TRAVERSAL_STACK. */
+int palz_photon_trace(const palz_scene_view* scene, const palz_photon_params* params,
+                      palz_photon_record* d_out, uint32_t* d_status, void* stream);
+
+/* Deterministic replacement of PathCache::store (pk.h:291-349): every photon
+ * tuple is inserted into the 2x2x2 cells nearest its landing point; entries
+ * are unique per (cell, event, triangles) and keep the smallest photon id
+ * (the reference keeps whichever photon arrives first). */
+typedef struct palz_tuple_table palz_tuple_table;
+int palz_tuple_table_create(palz_tuple_table** out);
+int palz_tuple_table_destroy(palz_tuple_table* t);
+int palz_tuple_table_build(palz_tuple_table* t, const palz_photon_record* d_records, uint32_t n_records,
+                           float cell_step, palz_sorter* sorter, void* stream);
+/* Synchronises and reports the table's sizes; any out-pointer may be NULL.
+ * Returns PALZ_ERR_CAPACITY if the build overflowed its key layout. */
+int palz_tuple_table_info(palz_tuple_table* t, uint32_t* n_valid, uint32_t* n_tuples,
+                          uint32_t* n_entries, uint32_t* n_cells);
+
+/* lookupFlatten + sort by path + reorderSpecularPath (main_gpu.cu:373-416,
+ * :515-525, pk.h:1486-1541): emit one palz_path per (receiver point, table
+ * entry in the point's cell) whose event matches `event` (0 = any event),
+ * grouped by tuple.  The count is written to *d_count (device); paths past
+ * `capacity` are dropped and the count still reports the full number. */
+int palz_lookup_flatten(palz_tuple_table* t, uint32_t event, const palz_first_diffuse* d_points,
+                        uint32_t n_points, uint32_t scene, palz_path* d_paths, uint32_t capacity,
+                        uint32_t* d_count, palz_sorter* sorter, void* stream);
+
+/* ---- specular path solvers ---------------------------------------------- */
+
+/* Everything a solver needs of one scene (device pointers).  The receiver
+ * irradiance of wavelength l at point i accumulates into
+ * image[l * n_points + i]: throughput * light_intensity / dAdw * cos, clamped
+ * at radiance_clamp (main_gpu.cu:458-483 without the CIE weighting).  The
+ * index of refraction is cauchy_a + cauchy_b / lambda^2, lambda in nm
+ * (CauchyDispersion, pk.h:935-949). */
+typedef struct {
+    const palz_triangle* tris;
+    const palz_triangle_attrib* attribs;
+    const palz_internal_node* nodes;
+    const palz_node_index* root;
+    const palz_first_diffuse* points;
+    const float* lambdas;
+    float* image;
+    uint32_t n_points;
+    uint32_t n_lambdas;
+    palz_float3 p_light;
+    float light_intensity;   /* m_lightIntencity = 16 (pk.h:1686) */
+    float radiance_clamp;    /* m_radianceClamp = 12 (pk.h:1688) */
+    float cauchy_a;
+    float cauchy_b;
+} palz_scene_desc;
+
+/* Per (path, wavelength) outcome, item index = path * n_lambdas + l. */
+enum { PALZ_PATH_NOT_CONVERGED = 0, PALZ_PATH_CONVERGED = 1, PALZ_PATH_CONTRIBUTES = 2 };
+typedef struct {
+    uint32_t status;
+    int32_t iterations;   /* Newton iterations run (index of the exit iteration) */
+    float contribution;   /* clamped irradiance added to the image */
+    float params[8];      /* solved barycentrics */
+    float footprint;      /* dA/dw of the contributing chain (pk.h:844-930), else 0 */
+} palz_path_result;
+
+/* solveSpecularPath_K{1..4} (main_gpu.cu:418-497): one thread per (path,
+ * wavelength).  `event` is the descriptor the path list was flattened for
+ * (its size must be K); every path whose own descriptor has K vertices is
+ * solved with that descriptor, others are skipped (status NOT_CONVERGED).
+ * `d_count` (optional, device) caps n_paths with a device-side count.
+ * `d_results` (optional) receives one record per item. */
+int palz_solve_specular(int K, uint32_t event, const palz_scene_desc* scene, const palz_path* d_paths,
+                        uint32_t n_paths, const uint32_t* d_count, palz_path_result* d_results,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

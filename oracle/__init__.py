@@ -26,6 +26,13 @@ _SIGS = {
     "orc_lbvh": (None, [_P, C.c_int, C.c_uint32, C.c_int, _P, _P, _P, _P, _P]),
     "orc_lbvh_topdown": (None, [_P, C.c_int, _P, _P, _P, _P]),
     "orc_validate_lbvh": (C.c_int, [_P, C.c_uint32, _P]),
+    "orc_sobol2d": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "orc_photon_trace": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, _P]),
+    "orc_lookup_paths": (C.c_uint32, [_P, C.c_uint32, C.c_float, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P,
+                                      C.c_uint32, _P]),
+    "orc_solve_paths": (None, [_P, _P, C.c_uint32, _P, C.c_int]),
+    "orc_solve_f64": (C.c_int, [_P, _P, _P, _P, _P, C.c_double, C.c_double, C.c_int, _P, _P]),
+    "orc_max_stack_depth": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
 }
 
 _lib = None
@@ -122,3 +129,88 @@ def morton_naive(x: int, y: int, z: int) -> int:
 
 def morton_splat(x: int, y: int, z: int) -> int:
     return int(lib().orc_morton_splat(x, y, z))
+
+
+# ---------------------------------------------------------------------------
+# caustic path (photon trace, tuple set, solver)
+
+def sobol2d(index: int, p0: int, p1: int, p2: int):
+    out = np.zeros(2, dtype=np.float32)
+    lib().orc_sobol2d(index, p0, p1, p2, _ptr(out))
+    return float(out[0]), float(out[1])
+
+
+def photon_trace(scene, nodes: np.ndarray, root: int, iteration: int = 0):
+    """All photon records of a CausticScene (host), plus the deepest stack use."""
+    from palzilla_amd import caustics as CA
+    recs = np.zeros(scene.n_tris * scene.photons_per_triangle, dtype=CA.PHOTON_RECORD)
+    pp = CA.PhotonParams(CA.Float3(*scene.p_light), scene.eta_min, scene.eta_max, iteration,
+                         CA.MIN_THROUGHPUT, scene.photons_per_triangle)
+    depth = lib().orc_photon_trace(_ptr(scene.tris), _ptr(scene.attribs), scene.n_tris,
+                                   _ptr(np.ascontiguousarray(nodes)), root, C.cast(C.pointer(pp), C.c_void_p),
+                                   _ptr(recs))
+    return recs, int(depth)
+
+
+def lookup_paths(records: np.ndarray, points: np.ndarray, event: int = 0, step: float = 0.01, scene_index: int = 0,
+                 capacity: int | None = None):
+    from palzilla_amd import caustics as CA
+    cap = capacity if capacity is not None else 1 << 22
+    out = np.zeros(cap, dtype=CA.PATH)
+    total = np.zeros(1, dtype=np.uint32)
+    n = lib().orc_lookup_paths(_ptr(records), records.shape[0], step, _ptr(points), points.shape[0], event,
+                               scene_index, _ptr(out), cap, _ptr(total))
+    if total[0] > cap:
+        return lookup_paths(records, points, event, step, scene_index, int(total[0]))
+    return out[:n]
+
+
+class HostScene:
+    """Host-side SceneDesc of a CausticScene for the oracle solver."""
+
+    def __init__(self, scene, nodes: np.ndarray, root: int):
+        from palzilla_amd import caustics as CA
+        self.scene = scene
+        self.nodes = np.ascontiguousarray(nodes)
+        self.root = np.array([root], dtype=np.uint32)
+        self.image = np.zeros(scene.points.shape[0] * scene.lambdas.shape[0], dtype=np.float32)
+        self.lambdas = scene.lambdas.astype(np.float32)
+        d = CA.SceneDesc()
+        d.tris = scene.tris.ctypes.data
+        d.attribs = scene.attribs.ctypes.data
+        d.nodes = self.nodes.ctypes.data
+        d.root = self.root.ctypes.data
+        d.points = scene.points.ctypes.data
+        d.lambdas = self.lambdas.ctypes.data
+        d.image = self.image.ctypes.data
+        d.n_points = scene.points.shape[0]
+        d.n_lambdas = scene.lambdas.shape[0]
+        d.p_light = CA.Float3(*scene.p_light)
+        d.light_intensity = CA.LIGHT_INTENSITY
+        d.radiance_clamp = CA.RADIANCE_CLAMP
+        d.cauchy_a = scene.cauchy_a
+        d.cauchy_b = scene.cauchy_b
+        self.desc = d
+
+
+def solve_paths(hs: HostScene, paths: np.ndarray, threads: int = 1, want_results: bool = True):
+    """FP32 oracle solve of every (path, wavelength); accumulates into hs.image."""
+    from palzilla_amd import caustics as CA
+    paths = np.ascontiguousarray(paths)
+    res = np.zeros(paths.shape[0] * hs.scene.lambdas.shape[0], dtype=CA.PATH_RESULT) if want_results else None
+    lib().orc_solve_paths(C.cast(C.pointer(hs.desc), C.c_void_p), _ptr(paths), paths.shape[0], _ptr(res), threads)
+    return res
+
+
+def solve_f64(scene, path, p_end, eta: float, params0, tol: float = 1e-28, max_iter: int = 64):
+    """FP64 Newton of one chain from params0; returns (converged, params, vertices (K,3))."""
+    K = int(path["event"]) >> 24
+    prm = np.zeros(8, dtype=np.float64)
+    prm[:2 * K] = params0[:2 * K]
+    verts = np.zeros(12, dtype=np.float64)
+    p = np.ascontiguousarray(np.array([path]))
+    pl = np.asarray(scene.p_light, dtype=np.float32)
+    pe = np.asarray(p_end, dtype=np.float32)
+    ok = lib().orc_solve_f64(_ptr(scene.tris), _ptr(scene.attribs), _ptr(p), _ptr(pl), _ptr(pe), float(eta), tol,
+                             max_iter, _ptr(prm), _ptr(verts))
+    return bool(ok), prm[:2 * K], verts[:3 * K].reshape(K, 3)

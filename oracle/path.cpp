@@ -1,0 +1,906 @@
+// Oracle: Sobol photons, photon tracing, the tuple set, the specular-chain
+// Newton solver (FP32 and FP64), path throughput, occlusion and dA/dw.
+// TEST INFRASTRUCTURE — a scalar restatement written from the reference text;
+// functions cite the reference lines they follow.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "oracle.hpp"
+
+using namespace orc;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// helper_math host semantics (helper_math.h:134-142)
+template <class T>
+T hfmin(T a, T b) { return a < b ? a : b; }
+template <class T>
+T hfmax(T a, T b) { return a > b ? a : b; }
+template <class T>
+vec3<T> normalize(vec3<T> v) { return v * (T(1) / std::sqrt(dot(v, v))); }  // rsqrtf = 1/sqrtf
+template <class T>
+T length(vec3<T> v) { return std::sqrt(dot(v, v)); }
+template <class T>
+vec3<T> cvt(const palz_float3& p) { return {(T)p.x, (T)p.y, (T)p.z}; }
+template <class T>
+vec3<T> cvt(float3 p) { return {(T)p.x, (T)p.y, (T)p.z}; }
+
+// ---------------------------------------------------------------------------
+// sobol.h:17-129
+uint32_t reverseBits(uint32_t v) {
+    v = ((v >> 1) & 0x55555555) | ((v & 0x55555555) << 1);
+    v = ((v >> 2) & 0x33333333) | ((v & 0x33333333) << 2);
+    v = ((v >> 4) & 0x0F0F0F0F) | ((v & 0x0F0F0F0F) << 4);
+    v = ((v >> 8) & 0x00FF00FF) | ((v & 0x00FF00FF) << 8);
+    v = (v >> 16) | (v << 16);
+    return v;
+}
+uint32_t laine_karras_permutation(uint32_t x, uint32_t seed) {
+    x += seed;
+    x ^= x * 0x6c50b47cu;
+    x ^= x * 0xb82f1e52u;
+    x ^= x * 0xc7afe638u;
+    x ^= x * 0x8d22f6e6u;
+    return x;
+}
+uint32_t nested_uniform_scramble(uint32_t x, uint32_t seed) {
+    x = reverseBits(x);
+    x = laine_karras_permutation(x, seed);
+    x = reverseBits(x);
+    return x;
+}
+uint32_t P(uint32_t v) {
+    v ^= v << 16;
+    v ^= (v & 0x00FF00FF) << 8;
+    v ^= (v & 0x0F0F0F0F) << 4;
+    v ^= (v & 0x33333333) << 2;
+    v ^= (v & 0x55555555) << 1;
+    return v;
+}
+float random_float(uint32_t u) { return (float)(u >> 8u) / ((0xFFFFFFFFu >> 8u) + 1u); }
+void pcg3d(uint32_t* x, uint32_t* y, uint32_t* z) {
+    uint32_t vx = *x, vy = *y, vz = *z;
+    vx = vx * 1664525u + 1013904223u;
+    vy = vy * 1664525u + 1013904223u;
+    vz = vz * 1664525u + 1013904223u;
+    vx += vy * vz;
+    vy += vz * vx;
+    vz += vx * vy;
+    vx ^= vx >> 16u;
+    vy ^= vy >> 16u;
+    vz ^= vz >> 16u;
+    vx += vy * vz;
+    vy += vz * vx;
+    vz += vx * vy;
+    *x = vx;
+    *y = vy;
+    *z = vz;
+}
+void shuffled_scrambled_sobol_2d(float* x, float* y, uint32_t index, uint32_t p0, uint32_t p1, uint32_t p2) {
+    pcg3d(&p0, &p1, &p2);
+    uint32_t v = laine_karras_permutation(reverseBits(index), p0);
+    *x = random_float(nested_uniform_scramble(v, p1));
+    *y = random_float(nested_uniform_scramble(P(v), p2));
+}
+void square2triangle(float* x, float* y) {  // pk.h:165-178
+    if (*y > *x) {
+        *x *= 0.5f;
+        *y -= *x;
+    } else {
+        *y *= 0.5f;
+        *x -= *y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// saka::dval (saka.h:11-208): one tangent, generic scalar
+template <class T>
+struct dval {
+    T v, g;
+    dval() : v(0), g(0) {}
+    dval(T x) : v(x), g(0) {}
+    dval(T x, bool rg) : v(x), g(rg ? T(1) : T(0)) {}
+};
+template <class T>
+dval<T> binary(dval<T> x, dval<T> y, T f, T dfdx, T dfdy) {
+    dval<T> u;
+    u.v = f;
+    u.g = x.g * dfdx + y.g * dfdy;
+    return u;
+}
+template <class T>
+dval<T> operator+(dval<T> x, dval<T> y) { return binary(x, y, x.v + y.v, T(1), T(1)); }
+template <class T>
+dval<T> operator-(dval<T> x) {
+    dval<T> u;
+    u.v = -x.v;
+    u.g = x.g * T(-1);
+    return u;
+}
+template <class T>
+dval<T> operator-(dval<T> x, dval<T> y) { return binary(x, y, x.v - y.v, T(1), T(-1)); }
+template <class T>
+dval<T> operator*(dval<T> x, dval<T> y) { return binary(x, y, x.v * y.v, y.v, x.v); }
+template <class T>
+dval<T> operator/(dval<T> x, dval<T> y) { return binary(x, y, x.v / y.v, T(1) / y.v, -x.v / (y.v * y.v)); }
+template <class T>
+dval<T> sqrt(dval<T> x) {
+    dval<T> u;
+    u.v = std::sqrt(x.v);
+    u.g = x.g * (T(0.5) / std::sqrt(x.v));
+    return u;
+}
+template <class T>
+struct dval3 {
+    dval<T> x, y, z;
+};
+template <class T>
+dval3<T> mkd(vec3<T> v) { return {dval<T>(v.x), dval<T>(v.y), dval<T>(v.z)}; }
+template <class T>
+dval3<T> operator+(dval3<T> a, dval3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T>
+dval3<T> operator-(dval3<T> a) { return {-a.x, -a.y, -a.z}; }
+template <class T>
+dval3<T> operator-(dval3<T> a, dval3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T>
+dval3<T> operator*(dval3<T> a, dval<T> s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T>
+dval3<T> operator/(dval3<T> a, dval<T> s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class T>
+dval<T> dot(dval3<T> a, dval3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T>
+dval3<T> cross(dval3<T> a, dval3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class T>
+dval3<T> reflection(dval3<T> wi, dval3<T> n) { return n * dot(wi, n) * dval<T>(T(2)) / dot(n, n) - wi; }
+template <class T>
+dval3<T> refraction_norm_free(dval3<T> wi, dval3<T> n, T eta) {
+    dval<T> NoN = dot(n, n);
+    dval<T> WIoN = dot(wi, n);
+    dval<T> WoW = dot(wi, wi);
+    dval<T> k = NoN * WoW * dval<T>(eta * eta - T(1)) + WIoN * WIoN;
+    if (k.v < T(0)) return {dval<T>(T(0)), dval<T>(T(0)), dval<T>(T(0))};
+    return -wi * NoN + n * (WIoN - sqrt(k));
+}
+
+// ---------------------------------------------------------------------------
+// sen.h: column-major dense matrices and Householder least squares
+template <class T, int R, int C>
+struct Mat {
+    T m[R * C];
+    T& operator()(int r, int c) { return m[c * R + r]; }
+    const T& operator()(int r, int c) const { return m[c * R + r]; }
+};
+template <class T, int R, int C>
+T column_dot(const Mat<T, R, C>& a, int c0, int c1) {  // sen.h:347-356
+    T r = T(0);
+    for (int i = 0; i < R; ++i) r += a.m[c0 * R + i] * a.m[c1 * R + i];
+    return r;
+}
+template <class T, int R, int C>
+void solve_qr_overdetermined(Mat<T, R, C> A, Mat<T, R, 1> b, T* x) {  // sen.h:658-747
+    constexpr int H = C < R - 1 ? C : R - 1;
+    Mat<T, R, H> vs;
+    for (int i = 0; i < H; i++) {
+        for (int j = 0; j < R; j++) vs(j, i) = j < i ? T(0) : -A(j, i);
+        T sgn = T(0) < vs(i, i) ? T(1) : T(-1);
+        T x_len = std::sqrt(column_dot(vs, i, i));
+        vs(i, i) += sgn * x_len;
+        T v_len2 = T(2) * x_len * (x_len + std::fabs(A(i, i)));
+        A(i, i) = sgn * x_len;
+        for (int r = i + 1; r < R; r++) A(r, i) = T(0);
+        for (int c = i + 1; c < C; c++) {
+            T VdotCol = T(0);
+            for (int r = i; r < R; r++) VdotCol += vs(r, i) * A(r, c);
+            T k = T(2) * VdotCol / v_len2;
+            for (int r = i; r < R; r++) A(r, c) -= k * vs(r, i);
+        }
+    }
+    for (int i = 0; i < H; i++) {  // applyQTransposed
+        T VdotCol = T(0), VdotV = T(0);
+        for (int r = i; r < R; r++) {
+            T v = vs(r, i);
+            VdotCol += v * b(r, 0);
+            VdotV += v * v;
+        }
+        T k = T(2) * VdotCol / VdotV;
+        for (int r = i; r < R; r++) b(r, 0) -= k * vs(r, i);
+    }
+    for (int r = C - 1; 0 <= r; r--) {
+        T sum = T(0);
+        for (int i = r + 1; i < C; i++) sum += A(r, i) * x[i];
+        x[r] = (b(r, 0) - sum) / A(r, r);
+    }
+}
+
+// ---------------------------------------------------------------------------
+struct Scene {
+    const palz_triangle* tris;
+    const palz_triangle_attrib* attribs;
+    const palz_internal_node* nodes;
+    uint32_t root;
+};
+
+bool is_T(uint32_t event, int k) { return (event >> k) & 1u; }
+
+// solveConstraints<K> (pk.h:466-592)
+template <int K, class T>
+bool solveConstraints(T* parameters, vec3<T> p_beg, vec3<T> p_end, const palz_triangle* tris,
+                      const palz_triangle_attrib* attribs, T eta, uint32_t event, int maxIterations, T costTolerance,
+                      int earlyTermWarmUp, int* iters) {
+    const int nParameters = K * 2;
+    T edge_eps = T(0.25);
+    for (int iter = 0; iter < maxIterations; iter++) {
+        Mat<T, K * 3, K * 2> A;
+        Mat<T, K * 3, 1> b;
+        T cost = T(0);
+        for (int i = 0; i < nParameters; i++) {
+            dval3<T> vertices[K + 2];
+            vertices[0] = mkd(p_beg);
+            vertices[K + 1] = mkd(p_end);
+            dval3<T> shadingNormals[K];
+            for (int k = 0; k < K; k++) {
+                dval<T> param_u(parameters[k * 2 + 0], k * 2 + 0 == i);
+                dval<T> param_v(parameters[k * 2 + 1], k * 2 + 1 == i);
+                // edges in the working precision (for T = float: the reference's float3 ops)
+                vec3<T> v0 = cvt<T>(tris[k].vs[0]), v1 = cvt<T>(tris[k].vs[1]), v2 = cvt<T>(tris[k].vs[2]);
+                vec3<T> e0 = v1 - v0, e1 = v2 - v0;
+                vertices[k + 1] = mkd(v0) + mkd(e0) * param_u + mkd(e1) * param_v;
+                vec3<T> n0 = cvt<T>(attribs[k].shading_normals[0]);
+                vec3<T> ne0 = cvt<T>(attribs[k].shading_normals[1]) - n0;
+                vec3<T> ne1 = cvt<T>(attribs[k].shading_normals[2]) - n0;
+                shadingNormals[k] = mkd(n0) + mkd(ne0) * param_u + mkd(ne1) * param_v;
+            }
+            for (int k = 0; k < K; k++) {
+                dval3<T> wi = vertices[k] - vertices[k + 1];
+                dval3<T> wo = vertices[k + 2] - vertices[k + 1];
+                dval3<T> n = shadingNormals[k];
+                if (dot(wi, n).v < T(0)) std::swap(wi, wo);
+                dval3<T> wo_optimize = is_T(event, k) ? refraction_norm_free(wi, n, eta) : reflection(wi, n);
+                dval3<T> c = cross(wo, wo_optimize);
+                A(k * 3 + 0, i) = c.x.g;
+                A(k * 3 + 1, i) = c.y.g;
+                A(k * 3 + 2, i) = c.z.g;
+                b(k * 3 + 0, 0) = c.x.v;
+                b(k * 3 + 1, 0) = c.y.v;
+                b(k * 3 + 2, 0) = c.z.v;
+                if (i == 0) cost += dot(c, c).v;
+            }
+        }
+        if (cost < costTolerance) {
+            *iters = iter;
+            return true;
+        }
+        T dparams[K * 2];
+        solve_qr_overdetermined(A, b, dparams);
+        for (int i = 0; i < nParameters; i++) parameters[i] = parameters[i] - dparams[i];
+        if (earlyTermWarmUp <= iter) {
+            for (int k = 0; k < K; k++) {
+                T param_u = parameters[k * 2 + 0];
+                T param_v = parameters[k * 2 + 1];
+                if (-edge_eps < param_u && -edge_eps < param_v && param_u + param_v < T(1) + edge_eps) continue;
+                *iters = iter;
+                return false;
+            }
+            edge_eps *= edge_eps;
+        }
+    }
+    *iters = maxIterations;
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// ray casting (minimum_lbvh.h:82-96, :372-422, :913-988, :1078-1152)
+float3 invRd(float3 rd) {
+    float3 q = {1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z};
+    const float M = 3.402823466e+38F;
+    return {hfmax(-M, hfmin(q.x, M)), hfmax(-M, hfmin(q.y, M)), hfmax(-M, hfmin(q.z, M))};
+}
+void slabs(float3 ro, float3 ird, float3 lo, float3 hi, float knownHitT, float* a, float* b) {
+    float3 t0 = {(lo.x - ro.x) * ird.x, (lo.y - ro.y) * ird.y, (lo.z - ro.z) * ird.z};
+    float3 t1 = {(hi.x - ro.x) * ird.x, (hi.y - ro.y) * ird.y, (hi.z - ro.z) * ird.z};
+    float3 tmin = {hfmin(t0.x, t1.x), hfmin(t0.y, t1.y), hfmin(t0.z, t1.z)};
+    float3 tmax = {hfmax(t0.x, t1.x), hfmax(t0.y, t1.y), hfmax(t0.z, t1.z)};
+    float region_min = hfmax(hfmax(tmin.x, tmin.y), tmin.z);
+    float region_max = hfmin(hfmin(tmax.x, tmax.y), tmax.z) * 1.00000024f;
+    *a = hfmax(region_min, 0.0f);
+    *b = hfmin(region_max, knownHitT);
+}
+bool intersectRayTriangle(float* tOut, float* uOut, float* vOut, float3* ngOut, float t_min, float t_max, float3 ro,
+                          float3 rd, float3 v0, float3 v1, float3 v2) {
+    float3 e0 = v1 - v0;
+    float3 e1 = v2 - v1;
+    float3 ng = cross(e0, e1);
+    float t = dot(v0 - ro, ng) / dot(ng, rd);
+    if (t_min <= t && t <= t_max) {
+        float3 e2 = v0 - v2;
+        float u_vol = dot(cross(e2, v0 + v2 - ro * 2.0f), rd);
+        float v_vol = dot(cross(e0, v1 + v0 - ro * 2.0f), rd);
+        float w_vol = dot(cross(e1, v2 + v1 - ro * 2.0f), rd);
+        if (hfmin(hfmin(u_vol, v_vol), w_vol) < 0.0f && 0.0f < hfmax(hfmax(u_vol, v_vol), w_vol)) return false;
+        float vol = u_vol + v_vol + w_vol;
+        *tOut = t;
+        *uOut = u_vol / vol;
+        *vOut = v_vol / vol;
+        *ngOut = ng;
+        return true;
+    }
+    return false;
+}
+struct Hit {
+    float t = 3.402823466e+38F;
+    float u = 0, v = 0;
+    float3 ng = {0, 0, 0};
+    uint32_t tri = 0xFFFFFFFFu;
+};
+constexpr uint32_t kLeaf = PALZ_NODE_LEAF_BIT;
+constexpr uint32_t kInvalid = PALZ_NODE_INVALID;
+
+// intersect_stackfull; *max_sp receives the deepest stack use
+void intersect_stackfull(Hit* hit, const Scene& s, float3 ro, float3 rd, float3 ird, int* max_sp) {
+    std::vector<uint32_t> stack;
+    uint32_t node = s.root;
+    while (node != kInvalid) {
+        if (node & kLeaf) {
+            const palz_triangle& tri = s.tris[node & ~kLeaf];
+            float t, u, v;
+            float3 ng;
+            if (intersectRayTriangle(&t, &u, &v, &ng, 0.0f, hit->t, ro, rd, load3(tri.vs[0]), load3(tri.vs[1]),
+                                     load3(tri.vs[2]))) {
+                hit->t = t;
+                hit->u = u;
+                hit->v = v;
+                hit->ng = ng;
+                hit->tri = node & ~kLeaf;
+            }
+            if (stack.empty()) {
+                node = kInvalid;
+            } else {
+                node = stack.back();
+                stack.pop_back();
+            }
+            continue;
+        }
+        const palz_internal_node& nd = s.nodes[node];
+        float l0, l1, r0, r1;
+        slabs(ro, ird, load3(nd.aabbs[0].lower), load3(nd.aabbs[0].upper), hit->t, &l0, &l1);
+        slabs(ro, ird, load3(nd.aabbs[1].lower), load3(nd.aabbs[1].upper), hit->t, &r0, &r1);
+        bool hitL = l0 <= l1, hitR = r0 <= r1;
+        if (hitL && hitR) {
+            uint32_t near_node = nd.children[0], far_node = nd.children[1];
+            if (r0 < l0) std::swap(near_node, far_node);
+            node = near_node;
+            stack.push_back(far_node);
+            if (max_sp && (int)stack.size() > *max_sp) *max_sp = (int)stack.size();
+        } else if (hitL || hitR) {
+            node = nd.children[hitL ? 0 : 1];
+        } else if (stack.empty()) {
+            node = kInvalid;
+        } else {
+            node = stack.back();
+            stack.pop_back();
+        }
+    }
+}
+
+// intersect_escape_link, any-hit (a leaf root is tested directly)
+bool any_hit(const Scene& s, float3 ro, float3 rd, float3 ird, float t_max) {
+    Hit hit;
+    hit.t = t_max;
+    float t, u, v;
+    float3 ng;
+    if (s.root & kLeaf) {
+        const palz_triangle& tri = s.tris[s.root & ~kLeaf];
+        return intersectRayTriangle(&t, &u, &v, &ng, 0.0f, hit.t, ro, rd, load3(tri.vs[0]), load3(tri.vs[1]),
+                                    load3(tri.vs[2]));
+    }
+    uint32_t curr = s.root, prev = kInvalid;
+    while (curr != kInvalid) {
+        if (curr & kLeaf) {
+            const palz_triangle& tri = s.tris[curr & ~kLeaf];
+            if (intersectRayTriangle(&t, &u, &v, &ng, 0.0f, hit.t, ro, rd, load3(tri.vs[0]), load3(tri.vs[1]),
+                                     load3(tri.vs[2])))
+                return true;
+            if (s.nodes[prev].children[0] == curr) {
+                std::swap(curr, prev);
+                continue;
+            }
+            uint32_t idx = s.nodes[prev].context;
+            prev = curr;
+            curr = idx != 0xFFFFFFFFu ? idx : kInvalid;
+            continue;
+        }
+        const palz_internal_node& nd = s.nodes[curr];
+        palz_aabb box;
+        uint32_t hitChild, miss;
+        if (prev == nd.parent) {
+            box = nd.aabbs[0];
+            hitChild = nd.children[0];
+            miss = curr;
+        } else {
+            box = nd.aabbs[1];
+            hitChild = nd.children[1];
+            miss = nd.context != 0xFFFFFFFFu ? nd.context : kInvalid;
+        }
+        float a, b;
+        slabs(ro, ird, load3(box.lower), load3(box.upper), hit.t, &a, &b);
+        prev = curr;
+        curr = a <= b ? hitChild : miss;
+    }
+    return false;
+}
+
+float rayOffsetScale(float3 p) {  // pk.h:43-49
+    p = {std::fabs(p.x), std::fabs(p.y), std::fabs(p.z)};
+    float maxElem = hfmax(hfmax(hfmax(p.x, p.y), p.z), 1.0f);
+    return maxElem * 1.192092896e-07F * 128.0f;
+}
+bool occluded(const Scene& s, float3 from, float3 from_n, float3 to, float3 to_n) {  // pk.h:88-115
+    if (dot(to - from, from_n) < 0.0f) from_n = -from_n;
+    if (dot(from - to, to_n) < 0.0f) to_n = -to_n;
+    float3 from_safe = from + from_n * rayOffsetScale(from);
+    float3 to_safe = to + to_n * rayOffsetScale(to);
+    float3 rd = to_safe - from_safe;
+    return any_hit(s, from_safe, rd, invRd(rd), 1.0f);
+}
+
+float fresnel_exact_norm_free(float3 wi, float3 n, float eta) {  // pk.h:427-450
+    float nn = dot(n, n);
+    float wiwi = dot(wi, wi);
+    float C = dot(n, wi);
+    if (C < 0.0f) {
+        C = -C;
+        n = -n;
+        eta = 1.0f / eta;
+    }
+    float K = nn * wiwi * (eta * eta - 1.0f) + C * C;
+    if (K < 0.0f) return 1.0f;
+    float G = std::sqrt(K);
+    auto sqr = [](float x) { return x * x; };
+    float GmC = G - C;
+    float GpC = G + C;
+    return 0.5f * sqr(GmC / GpC) * (1.0f + sqr((C * GpC - nn * wiwi) / (C * GmC + nn * wiwi)));
+}
+bool refraction_norm_free(float3* wo, float3 wi, float3 n, float eta) {  // pk.h:181-200
+    float NoN = dot(n, n);
+    float WIoN = dot(wi, n);
+    if (WIoN < 0.0f) {
+        WIoN = -WIoN;
+        n = -n;
+        eta = 1.0f / eta;
+    }
+    float WIoWI = dot(wi, wi);
+    float k = NoN * WIoWI * (eta * eta - 1.0f) + WIoN * WIoN;
+    if (k < 0.0f) return false;
+    *wo = -wi * NoN + n * (WIoN - std::sqrt(k));
+    return true;
+}
+float3 reflection(float3 wi, float3 n) { return n * dot(wi, n) * 2.0f / dot(n, n) - wi; }  // pk.h:202-205
+
+float3 getVertex(const palz_triangle& tri, float u, float v) {  // pk.h:752-758
+    float3 v0 = load3(tri.vs[0]);
+    float3 e0 = load3(tri.vs[1]) - v0;
+    float3 e1 = load3(tri.vs[2]) - v0;
+    return v0 + e0 * u + e1 * v;
+}
+
+// contributableThroughput<K> (pk.h:760-842)
+template <int K>
+float contributableThroughput(const float* parameters, float3 p_beg, float3 p_end, const palz_triangle* tris,
+                              const palz_triangle_attrib* attribs, uint32_t event, const Scene& s, float eta) {
+    float3 vertices[K + 2];
+    vertices[0] = p_beg;
+    vertices[K + 1] = p_end;
+    float3 shadingNormals[K + 2];
+    for (int k = 0; k < K; k++) {
+        float param_u = parameters[k * 2 + 0];
+        float param_v = parameters[k * 2 + 1];
+        if (param_u < 0.0f || param_v < 0.0f || 1.0f < param_u + param_v) return 0.0f;
+        vertices[k + 1] = getVertex(tris[k], param_u, param_v);
+        float3 n0 = load3(attribs[k].shading_normals[0]);
+        float3 ne0 = load3(attribs[k].shading_normals[1]) - n0;
+        float3 ne1 = load3(attribs[k].shading_normals[2]) - n0;
+        shadingNormals[k + 1] = normalize(n0 + ne0 * param_u + ne1 * param_v);
+    }
+    shadingNormals[0] = normalize(vertices[1] - vertices[0]);
+    shadingNormals[K + 1] = normalize(vertices[K] - vertices[K + 1]);
+    float throughput = 1.0f;
+    for (int k = 0; k < K; k++) {
+        float3 wi = vertices[k] - vertices[k + 1];
+        float3 wo = vertices[k + 2] - vertices[k + 1];
+        float3 n = shadingNormals[k + 1];
+        bool eT = !(0.0f < dot(wi, n) * dot(wo, n));
+        if (is_T(event, k) != eT) return 0.0f;
+        float3 v0 = load3(tris[k].vs[0]);
+        float3 ng = cross(load3(tris[k].vs[1]) - v0, load3(tris[k].vs[2]) - v0);
+        bool egT = !(0.0f < dot(wi, ng) * dot(wo, ng));
+        if (eT != egT) return 0.0f;
+        if (attribs[k].material == PALZ_MATERIAL_DIELECTRIC) {
+            float reflectance = fresnel_exact_norm_free(wi, n, eta);
+            throughput *= !eT ? reflectance : 1.0f - reflectance;
+        }
+    }
+    for (int i = 0; i < K + 1; i++) {
+        float3 d = vertices[i] - vertices[i + 1];
+        if (dot(d, d) < 1.0e-5f) return 0.0f;
+    }
+    for (int i = 0; i < K + 1; i++)
+        if (occluded(s, vertices[i], shadingNormals[i], vertices[i + 1], shadingNormals[i + 1])) return 0.0f;
+    return throughput;
+}
+
+void GetOrthonormalBasis(float3 zaxis, float3* xaxis, float3* yaxis) {  // pk.h:453-459
+    const float sign = std::copysign(1.0f, zaxis.z);
+    const float a = -1.0f / (sign + zaxis.z);
+    const float b = zaxis.x * zaxis.y * a;
+    *xaxis = {1.0f + sign * zaxis.x * zaxis.x * a, sign * b, -sign * zaxis.x};
+    *yaxis = {b, sign + zaxis.y * zaxis.y * a, -zaxis.y};
+}
+dval3<float> intersect_p_ray_plane(dval3<float> ro, dval3<float> rd, dval3<float> ng, dval3<float> v0) {
+    auto t = dot(v0 - ro, ng) / dot(ng, rd);
+    return ro + rd * t;
+}
+
+// dAdw (pk.h:844-930)
+float dAdw(float3 ro, float3 rd, float3 p_end, const palz_triangle* tris, const palz_triangle_attrib* attribs,
+           uint32_t event, int size, float eta) {
+    rd = normalize(rd);
+    float3 dAxis[2];
+    for (int i = 0; i < 2; i++) {
+        dval<float> differentials[2] = {dval<float>(0.0f), dval<float>(0.0f)};
+        differentials[i].g = 1.0f;
+        dval3<float> ro_j = mkd(ro);
+        float3 T0, T1;
+        GetOrthonormalBasis(rd, &T0, &T1);
+        dval3<float> rd_j = mkd(rd) + mkd(T0) * differentials[0] + mkd(T1) * differentials[1];
+        for (int j = 0; j < size; j++) {
+            float3 v0 = load3(tris[j].vs[0]), v1 = load3(tris[j].vs[1]), v2 = load3(tris[j].vs[2]);
+            float3 ng = cross(v1 - v0, v2 - v0);
+            dval3<float> p = intersect_p_ray_plane(ro_j, rd_j, mkd(ng), mkd(v0));
+            dval<float> v = dot(p - mkd(v1), mkd(cross(ng, v1 - v0)));
+            dval<float> w = dot(p - mkd(v2), mkd(cross(ng, v2 - v1)));
+            dval<float> u = dot(p - mkd(v0), mkd(cross(ng, v0 - v2)));
+            dval<float> area = u + v + w;
+            u = u / area;
+            v = v / area;
+            w = w / area;
+            dval3<float> n = mkd(load3(attribs[j].shading_normals[0])) * w +
+                             mkd(load3(attribs[j].shading_normals[1])) * u +
+                             mkd(load3(attribs[j].shading_normals[2])) * v;
+            dval3<float> wi = -rd_j;
+            dval3<float> wo;
+            if (is_T(event, j)) {
+                if (dot(wi, n).v < 0.0f) {
+                    wo = refraction_norm_free(wi, -n, 1.0f / eta);
+                    if (wo.x.v == 0.0f && wo.y.v == 0.0f && wo.z.v == 0.0f) return 3.402823466e+38f;
+                } else {
+                    wo = refraction_norm_free(wi, n, eta);
+                }
+            } else {
+                wo = reflection(wi, n);
+            }
+            ro_j = p;
+            rd_j = wo;
+        }
+        float3 p_last = {ro_j.x.v, ro_j.y.v, ro_j.z.v};
+        dval3<float> p_final = intersect_p_ray_plane(ro_j, rd_j, mkd(p_last - p_end), mkd(p_end));
+        dAxis[i] = {p_final.x.g, p_final.y.g, p_final.z.g};
+    }
+    float len0 = length(dAxis[0]);
+    float len1 = length(dAxis[1]);
+    float sinTheta = length(cross(dAxis[0] / len0, dAxis[1] / len1));
+    return len0 * len1 * hfmax(sinTheta, 0.01f);
+}
+
+struct SolveCtx {
+    Scene s;
+    const palz_first_diffuse* points;
+    const float* lambdas;
+    uint32_t n_lambdas;
+    float3 p_light;
+    float light_intensity, radiance_clamp, cauchy_a, cauchy_b;
+};
+
+// solveSpecularPath<K> (main_gpu.cu:418-486) for one (path, wavelength)
+template <int K>
+void solve_item(const SolveCtx& c, const palz_path& path, uint32_t l, palz_path_result* res) {
+    palz_triangle tris[K];
+    palz_triangle_attrib attribs[K];
+    for (int k = 0; k < K; k++) {
+        tris[k] = c.s.tris[path.tris[k]];
+        attribs[k] = c.s.attribs[path.tris[k]];
+    }
+    float parameters[K * 2];
+    for (int i = 0; i < K * 2; i++) parameters[i] = path.params[i];
+    const palz_first_diffuse& q = c.points[path.pixel];
+    float3 p = load3(q.p), n = load3(q.ng);
+    float lambda = c.lambdas[l];
+    float eta = c.cauchy_a + c.cauchy_b / (lambda * lambda);
+    int iters = 0;
+    bool converged = solveConstraints<K, float>(parameters, c.p_light, p, tris, attribs, eta, path.event, 32, 1.0e-10f,
+                                                1, &iters);
+    std::memset(res, 0, sizeof(*res));
+    res->iterations = iters;
+    if (converged) {
+        res->status = PALZ_PATH_CONVERGED;
+        float throughput =
+            contributableThroughput<K>(parameters, c.p_light, p, tris, attribs, path.event, c.s, eta);
+        if (0.0f < throughput) {
+            float dAdwValue = dAdw(c.p_light, getVertex(tris[0], parameters[0], parameters[1]) - c.p_light, p, tris,
+                                   attribs, path.event, K, eta);
+            float3 last = getVertex(tris[K - 1], parameters[2 * (K - 1)], parameters[2 * (K - 1) + 1]);
+            float contrib = throughput * c.light_intensity / dAdwValue * hfmax(dot(normalize(last - p), n), 0.0f);
+            contrib = hfmin(contrib, c.radiance_clamp);
+            res->status = PALZ_PATH_CONTRIBUTES;
+            res->contribution = contrib;
+            res->footprint = dAdwValue;
+        }
+    }
+    for (int i = 0; i < 8; i++) res->params[i] = i < 2 * K ? parameters[i] : 0.0f;
+}
+
+void solve_any(const SolveCtx& c, const palz_path& path, uint32_t l, palz_path_result* res) {
+    switch (path.event >> 24) {
+        case 1: solve_item<1>(c, path, l, res); break;
+        case 2: solve_item<2>(c, path, l, res); break;
+        case 3: solve_item<3>(c, path, l, res); break;
+        case 4: solve_item<4>(c, path, l, res); break;
+        default: std::memset(res, 0, sizeof(*res));
+    }
+}
+
+SolveCtx make_ctx(const palz_scene_desc* sd) {
+    SolveCtx c;
+    c.s = {sd->tris, sd->attribs, sd->nodes, *sd->root};
+    c.points = sd->points;
+    c.lambdas = sd->lambdas;
+    c.n_lambdas = sd->n_lambdas;
+    c.p_light = load3(sd->p_light);
+    c.light_intensity = sd->light_intensity;
+    c.radiance_clamp = sd->radiance_clamp;
+    c.cauchy_a = sd->cauchy_a;
+    c.cauchy_b = sd->cauchy_b;
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_sobol2d(uint32_t index, uint32_t p0, uint32_t p1, uint32_t p2, float* out) {
+    shuffled_scrambled_sobol_2d(out, out + 1, index, p0, p1, p2);
+}
+
+// photonTrace (main_gpu.cu:56-211), all photon slots serially.  Host pointers;
+// the scene's root is a host value.  Returns the deepest traversal stack use.
+int orc_photon_trace(const palz_triangle* tris, const palz_triangle_attrib* attribs, uint32_t n_tris,
+                     const palz_internal_node* nodes, uint32_t root, const palz_photon_params* pp,
+                     palz_photon_record* out) {
+    Scene s{tris, attribs, nodes, root};
+    int max_sp = 0;
+    const uint32_t Pn = pp->photons_per_triangle;
+    const float3 p_light = load3(pp->p_light);
+    for (uint32_t iTri = 0; iTri < n_tris; iTri++) {
+        for (uint32_t j = 0; j < Pn; j++) {
+            palz_photon_record& rec = out[(size_t)iTri * Pn + j];
+            std::memset(&rec, 0, sizeof(rec));
+            rec.photon = iTri * Pn + j;
+            for (int k = 0; k < 4; k++) rec.tris[k] = -1;
+            if (attribs[iTri].material == PALZ_MATERIAL_DIFFUSE) continue;
+            const palz_triangle& tri = tris[iTri];
+            float3 v0 = load3(tri.vs[0]);
+            float3 ng = cross(load3(tri.vs[1]) - v0, load3(tri.vs[2]) - v0);
+            float3 ns = load3(attribs[iTri].shading_normals[0]);
+            ng = ng * dot(ng, ns);
+            if (dot(ng, p_light - v0) < 0.0f) continue;
+            float px, py;
+            shuffled_scrambled_sobol_2d(&px, &py, j, (uint32_t)pp->iteration, iTri, 789);
+            square2triangle(&px, &py);
+            float ex, ey;
+            shuffled_scrambled_sobol_2d(&ex, &ey, j, (uint32_t)pp->iteration, iTri, 178);
+            float eta = pp->eta_min + ex * (pp->eta_max - pp->eta_min);
+            float3 e0 = load3(tri.vs[1]) - v0, e1 = load3(tri.vs[2]) - v0;
+            float3 p = v0 + e0 * px + e1 * py;
+            float3 ro = p_light, rd = p - p_light;
+            uint32_t events = 0;
+            int size = 0;
+            bool admissiblePath = false;
+            int triIndices[4];
+            float parameters[8];
+            float3 p_final = {0, 0, 0};
+            float throughtput = 1.0f;
+            for (int d = 0; d < 4 + 1; d++) {
+                Hit hit;
+                intersect_stackfull(&hit, s, ro, rd, invRd(rd), &max_sp);
+                if (hit.t == 3.402823466e+38F) break;
+                const palz_triangle_attrib& attrib = attribs[hit.tri];
+                float3 p_hit = ro + rd * hit.t;
+                float3 a0 = load3(attrib.shading_normals[0]);
+                float3 nsh = a0 + (load3(attrib.shading_normals[1]) - a0) * hit.u +
+                             (load3(attrib.shading_normals[2]) - a0) * hit.v;
+                if (attrib.material == PALZ_MATERIAL_DIFFUSE) {
+                    if (d == 0) break;
+                    admissiblePath = true;
+                    p_final = p_hit;
+                    break;
+                }
+                if (d == 4) break;
+                float3 wi = -rd;
+                triIndices[d] = (int)hit.tri;
+                parameters[d * 2] = hit.u;
+                parameters[d * 2 + 1] = hit.v;
+                float rx, ry;
+                shuffled_scrambled_sobol_2d(&rx, &ry, j, (uint32_t)d, iTri, (uint32_t)pp->iteration);
+                float reflectance = fresnel_exact_norm_free(wi, nsh, eta);
+                bool eT = false;
+                if (attrib.material == PALZ_MATERIAL_DIELECTRIC) eT = !(rx < reflectance);
+                if (eT) events |= 1u << size;
+                size++;
+                float sign = 1.0f;
+                float3 wo;
+                if (!eT) {
+                    wo = reflection(wi, nsh);
+                    if (attrib.material == PALZ_MATERIAL_DIELECTRIC) throughtput *= reflectance;
+                } else {
+                    if (!refraction_norm_free(&wo, wi, nsh, eta)) break;
+                    sign = -1.0f;
+                    throughtput *= 1.0f - reflectance;
+                }
+                if (0.0f < dot(hit.ng, wi) * dot(hit.ng, wo) * sign) {
+                    float3 offset_dir = normalize(hit.ng * dot(hit.ng, wo));
+                    ro = p_hit + offset_dir * rayOffsetScale(p_hit);
+                    rd = wo;
+                    continue;
+                }
+                break;
+            }
+            if (admissiblePath && pp->min_throughput < throughtput) {
+                rec.event = ((uint32_t)size << 24) | events;
+                for (int k = 0; k < size; k++) {
+                    rec.tris[k] = triIndices[k];
+                    rec.params[2 * k] = parameters[2 * k];
+                    rec.params[2 * k + 1] = parameters[2 * k + 1];
+                }
+                rec.p_final = {p_final.x, p_final.y, p_final.z};
+                rec.throughput = throughtput;
+            }
+        }
+    }
+    return max_sp;
+}
+
+// The tuple set and the path list, by direct set construction (std::map):
+// for each valid photon in id order, insert (cell, event, tris) for its 2x2x2
+// nearest cells (pk.h:302-313) if absent; each receiver point then lists the
+// entries of its own cell (pk.h:352-373) whose event matches.  Output paths
+// are in (point, event, tris) order; returns the number written (<= cap),
+// *total receives the full count.
+uint32_t orc_lookup_paths(const palz_photon_record* rec, uint32_t n_rec, float step, const palz_first_diffuse* pts,
+                          uint32_t n_pts, uint32_t event, uint32_t scene, palz_path* out, uint32_t cap,
+                          uint32_t* total) {
+    using Key = std::tuple<int, int, int, uint32_t, int, int, int, int>;
+    std::map<Key, uint32_t> table;  // -> photon slot of the first (smallest id) photon
+    for (uint32_t i = 0; i < n_rec; i++) {
+        const palz_photon_record& r = rec[i];
+        if (r.event == 0) continue;
+        float fx = r.p_final.x / step, fy = r.p_final.y / step, fz = r.p_final.z / step;
+        int x = (int)std::floor(fx), y = (int)std::floor(fy), z = (int)std::floor(fz);
+        int dx = fx - x < 0.5f ? -1 : 1, dy = fy - y < 0.5f ? -1 : 1, dz = fz - z < 0.5f ? -1 : 1;
+        int size = (int)(r.event >> 24);
+        int t[4] = {-1, -1, -1, -1};
+        for (int k = 0; k < size; k++) t[k] = r.tris[k];
+        for (int iz = 0; iz < 2; iz++)
+            for (int iy = 0; iy < 2; iy++)
+                for (int ix = 0; ix < 2; ix++) {
+                    Key key{x + ix * dx, y + iy * dy, z + iz * dz, r.event, t[0], t[1], t[2], t[3]};
+                    table.emplace(key, i);
+                }
+    }
+    uint32_t n = 0;
+    for (uint32_t pi = 0; pi < n_pts; pi++) {
+        const palz_first_diffuse& q = pts[pi];
+        if (q.R.x == 0.0f && q.R.y == 0.0f && q.R.z == 0.0f) continue;
+        int x = (int)std::floor(q.p.x / step), y = (int)std::floor(q.p.y / step), z = (int)std::floor(q.p.z / step);
+        auto it = table.lower_bound(Key{x, y, z, 0u, INT32_MIN, INT32_MIN, INT32_MIN, INT32_MIN});
+        for (; it != table.end(); ++it) {
+            const Key& k = it->first;
+            if (std::get<0>(k) != x || std::get<1>(k) != y || std::get<2>(k) != z) break;
+            if (event != 0 && std::get<3>(k) != event) continue;
+            if (n < cap) {
+                const palz_photon_record& r = rec[it->second];
+                palz_path& pa = out[n];
+                pa.scene = scene;
+                pa.pixel = pi;
+                pa.event = r.event;
+                pa.photon = r.photon;
+                for (int t = 0; t < 4; t++) pa.tris[t] = r.tris[t];
+                for (int t = 0; t < 8; t++) pa.params[t] = r.params[t];
+            }
+            n++;
+        }
+    }
+    *total = n;
+    return std::min(n, cap);
+}
+
+// Plain FP32 solve of every (path, wavelength) with `threads` host threads
+// (ParallelFor, main.cpp:1745); adds contributions into sd->image (host) in
+// item order per thread range, and writes results if not NULL.
+void orc_solve_paths(const palz_scene_desc* sd, const palz_path* paths, uint32_t n_paths, palz_path_result* results,
+                     int threads) {
+    SolveCtx c = make_ctx(sd);
+    const uint64_t items = (uint64_t)n_paths * c.n_lambdas;
+    std::vector<palz_path_result> local(results ? 0 : items);
+    palz_path_result* res = results ? results : local.data();
+    threads = std::max(1, threads);
+    auto work = [&](uint64_t b, uint64_t e) {
+        for (uint64_t it = b; it < e; it++) solve_any(c, paths[it / c.n_lambdas], (uint32_t)(it % c.n_lambdas), &res[it]);
+    };
+    if (threads == 1) {
+        work(0, items);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; t++)
+            pool.emplace_back(work, items * t / threads, items * (t + 1) / threads);
+        for (auto& th : pool) th.join();
+    }
+    for (uint64_t it = 0; it < items; it++)
+        if (res[it].status == PALZ_PATH_CONTRIBUTES)
+            sd->image[(size_t)(it % c.n_lambdas) * sd->n_points + paths[it / c.n_lambdas].pixel] += res[it].contribution;
+}
+
+// FP64 Newton of one chain from `params` (FP32 start), to cost < tol64, with
+// the reference's iteration structure in double precision.  Writes the
+// solution and its K vertices (x,y,z per vertex).  Returns 1 if converged.
+int orc_solve_f64(const palz_triangle* tris_all, const palz_triangle_attrib* attribs_all, const palz_path* path,
+                  const float* p_light, const float* p_end, double eta, double tol64, int max_iter, double* params_out,
+                  double* vertices_out) {
+    const int K = (int)(path->event >> 24);
+    palz_triangle tris[4];
+    palz_triangle_attrib attribs[4];
+    for (int k = 0; k < K; k++) {
+        tris[k] = tris_all[path->tris[k]];
+        attribs[k] = attribs_all[path->tris[k]];
+    }
+    double prm[8];
+    for (int i = 0; i < 2 * K; i++) prm[i] = params_out[i];
+    double3 pb = {p_light[0], p_light[1], p_light[2]}, pe = {p_end[0], p_end[1], p_end[2]};
+    int iters = 0;
+    bool ok = false;
+    switch (K) {
+        case 1: ok = solveConstraints<1, double>(prm, pb, pe, tris, attribs, eta, path->event, max_iter, tol64, max_iter, &iters); break;
+        case 2: ok = solveConstraints<2, double>(prm, pb, pe, tris, attribs, eta, path->event, max_iter, tol64, max_iter, &iters); break;
+        case 3: ok = solveConstraints<3, double>(prm, pb, pe, tris, attribs, eta, path->event, max_iter, tol64, max_iter, &iters); break;
+        case 4: ok = solveConstraints<4, double>(prm, pb, pe, tris, attribs, eta, path->event, max_iter, tol64, max_iter, &iters); break;
+        default: return 0;
+    }
+    for (int i = 0; i < 2 * K; i++) params_out[i] = prm[i];
+    for (int k = 0; k < K; k++) {
+        double3 v0 = {tris[k].vs[0].x, tris[k].vs[0].y, tris[k].vs[0].z};
+        double3 v1 = {tris[k].vs[1].x, tris[k].vs[1].y, tris[k].vs[1].z};
+        double3 v2 = {tris[k].vs[2].x, tris[k].vs[2].y, tris[k].vs[2].z};
+        double3 x = v0 + (v1 - v0) * prm[2 * k] + (v2 - v0) * prm[2 * k + 1];
+        vertices_out[3 * k + 0] = x.x;
+        vertices_out[3 * k + 1] = x.y;
+        vertices_out[3 * k + 2] = x.z;
+    }
+    return ok ? 1 : 0;
+}
+
+int orc_max_stack_depth(const palz_triangle* tris, const palz_internal_node* nodes, uint32_t root, const float* ro,
+                        const float* rd) {
+    Scene s{tris, nullptr, nodes, root};
+    Hit hit;
+    int m = 0;
+    float3 o = {ro[0], ro[1], ro[2]}, d = {rd[0], rd[1], rd[2]};
+    intersect_stackfull(&hit, s, o, d, invRd(d), &m);
+    return m;
+}
+
+}  // extern "C"

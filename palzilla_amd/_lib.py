@@ -54,6 +54,16 @@ _SIGS = {
     "palz_lbvh_result": (C.c_int, [C.c_void_p, c_void_pp, c_void_pp, c_void_pp,
                                    C.POINTER(C.c_int)]),
     "palz_lbvh_sorted": (C.c_int, [C.c_void_p, c_void_pp, c_void_pp]),
+    "palz_photon_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "palz_tuple_table_create": (C.c_int, [c_void_pp]),
+    "palz_tuple_table_destroy": (C.c_int, [C.c_void_p]),
+    "palz_tuple_table_build": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_void_p,
+                                         C.c_void_p]),
+    "palz_tuple_table_info": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_uint32)] * 4),
+    "palz_lookup_flatten": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                      C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "palz_solve_specular": (C.c_int, [C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]),
     "palz_lbvh_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "palz_lbvh_timings": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int,
                                     C.POINTER(C.c_int)]),

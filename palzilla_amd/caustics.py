@@ -1,0 +1,194 @@
+"""Host-side mirror of the reference's caustic step (PKRenderer::loadFrame and
+PKRenderer::step, pk.h:1246-1627) over the C ABI.
+
+A `GpuScene` owns the device copies of one CausticScene and its LBVH.  The
+methods follow the reference's order: photonTrace -> PathCache (here the
+deterministic tuple table) -> lookupFlatten / sort / reorder per event
+descriptor -> solveSpecularPath_K.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import types as T
+from ._lib import call, lib
+from .device import DeviceBuffer, _sp, download_ptr
+from .lbvh import BVHGPUBuilder, OnesweepSort
+from .scenes import FIRST_DIFFUSE, CausticScene
+
+
+class Float3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class SceneView(C.Structure):
+    _fields_ = [("tris", C.c_void_p), ("attribs", C.c_void_p), ("nodes", C.c_void_p), ("root", C.c_void_p),
+                ("n_tris", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+class PhotonParams(C.Structure):
+    _fields_ = [("p_light", Float3), ("eta_min", C.c_float), ("eta_max", C.c_float), ("iteration", C.c_int32),
+                ("min_throughput", C.c_float), ("photons_per_triangle", C.c_uint32)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("tris", C.c_void_p), ("attribs", C.c_void_p), ("nodes", C.c_void_p), ("root", C.c_void_p),
+                ("points", C.c_void_p), ("lambdas", C.c_void_p), ("image", C.c_void_p),
+                ("n_points", C.c_uint32), ("n_lambdas", C.c_uint32), ("p_light", Float3),
+                ("light_intensity", C.c_float), ("radiance_clamp", C.c_float), ("cauchy_a", C.c_float),
+                ("cauchy_b", C.c_float)]
+
+
+assert C.sizeof(SceneView) == 40
+assert C.sizeof(PhotonParams) == 32
+assert C.sizeof(SceneDesc) == 92 or C.sizeof(SceneDesc) == 96
+
+PHOTON_RECORD = np.dtype([("photon", "<u4"), ("event", "<u4"), ("tris", "<i4", 4), ("params", "<f4", 8),
+                          ("p_final", "<f4", 3), ("throughput", "<f4")])
+assert PHOTON_RECORD.itemsize == 72
+PATH = np.dtype([("scene", "<u4"), ("pixel", "<u4"), ("event", "<u4"), ("photon", "<u4"), ("tris", "<i4", 4),
+                 ("params", "<f4", 8)])
+assert PATH.itemsize == 64
+PATH_RESULT = np.dtype([("status", "<u4"), ("iterations", "<i4"), ("contribution", "<f4"), ("params", "<f4", 8),
+                        ("footprint", "<f4")])
+assert PATH_RESULT.itemsize == 48
+
+LIGHT_INTENSITY = 16.0   # m_lightIntencity (pk.h:1686)
+RADIANCE_CLAMP = 12.0    # m_radianceClamp (pk.h:1688)
+MIN_THROUGHPUT = 0.05    # m_minThroughput (pk.h:1687)
+CELL_STEP = 0.01         # m_pathCache.init(0.01f) (pk.h:1376)
+
+
+def event_u32(events: str) -> int:
+    """EventDescriptor::asU32 for a string of R/T events, e.g. "TT"."""
+    bits = 0
+    for i, e in enumerate(events):
+        if e == "T":
+            bits |= 1 << i
+    return (len(events) << 24) | bits
+
+
+def event_str(ev: int) -> str:
+    return "".join("T" if (ev >> i) & 1 else "R" for i in range(ev >> 24))
+
+
+class TupleTable:
+    def __init__(self):
+        h = C.c_void_p()
+        call("palz_tuple_table_create", C.byref(h))
+        self.handle = h.value
+
+    def build(self, records: int, n_records: int, step: float, sorter: OnesweepSort, stream=None):
+        call("palz_tuple_table_build", C.c_void_p(self.handle), C.c_void_p(records), C.c_uint32(n_records),
+             C.c_float(step), C.c_void_p(sorter.handle), _sp(stream))
+
+    def info(self) -> dict:
+        v = [C.c_uint32(0) for _ in range(4)]
+        call("palz_tuple_table_info", C.c_void_p(self.handle), *[C.byref(x) for x in v])
+        return dict(zip(("n_valid", "n_tuples", "n_entries", "n_cells"), (x.value for x in v)))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().palz_tuple_table_destroy(C.c_void_p(self.handle))
+                self.handle = None
+        except Exception:
+            pass
+
+
+class GpuScene:
+    """Device copies of a CausticScene plus its LBVH and working buffers."""
+
+    def __init__(self, scene: CausticScene, sorter: OnesweepSort | None = None, stream=None):
+        self.scene = scene
+        self.stream = stream
+        self.sorter = sorter or OnesweepSort()
+        self.d_tris = DeviceBuffer.from_array(scene.tris)
+        self.d_attribs = DeviceBuffer.from_array(scene.attribs)
+        self.d_points = DeviceBuffer.from_array(scene.points)
+        self.d_lambdas = DeviceBuffer.from_array(scene.lambdas.astype(np.float32))
+        self.n_points = scene.points.shape[0]
+        self.n_lambdas = scene.lambdas.shape[0]
+        self.d_image = DeviceBuffer(4 * self.n_points * self.n_lambdas)
+        self.d_image.zero(stream)
+        self.bvh = BVHGPUBuilder()
+        self.bvh.build(self.d_tris.ptr, scene.n_tris, 0, self.sorter, stream=stream, morton_bits=32)
+        self.table = TupleTable()
+        self.d_records = None
+        self.d_status = DeviceBuffer(16)
+        self.d_status.zero(stream)
+        self.d_count = DeviceBuffer(16)
+
+    # -- C ABI records ---------------------------------------------------------
+    def view(self) -> SceneView:
+        return SceneView(self.d_tris.ptr, self.d_attribs.ptr, self.bvh.m_internals, self.bvh.m_rootNode,
+                         self.scene.n_tris, 0)
+
+    def desc(self, image_ptr: int | None = None) -> SceneDesc:
+        s = self.scene
+        return SceneDesc(self.d_tris.ptr, self.d_attribs.ptr, self.bvh.m_internals, self.bvh.m_rootNode,
+                         self.d_points.ptr, self.d_lambdas.ptr, image_ptr or self.d_image.ptr, self.n_points,
+                         self.n_lambdas, Float3(*s.p_light), LIGHT_INTENSITY, RADIANCE_CLAMP, s.cauchy_a, s.cauchy_b)
+
+    def photon_params(self, iteration: int = 0) -> PhotonParams:
+        s = self.scene
+        return PhotonParams(Float3(*s.p_light), s.eta_min, s.eta_max, iteration, MIN_THROUGHPUT,
+                            s.photons_per_triangle)
+
+    @property
+    def n_photon_slots(self) -> int:
+        return self.scene.n_tris * self.scene.photons_per_triangle
+
+    # -- pipeline --------------------------------------------------------------
+    def trace(self, iteration: int = 0) -> None:
+        n = self.n_photon_slots
+        if self.d_records is None or self.d_records.nbytes < n * PHOTON_RECORD.itemsize:
+            self.d_records = DeviceBuffer(n * PHOTON_RECORD.itemsize)
+        v, p = self.view(), self.photon_params(iteration)
+        call("palz_photon_trace", C.byref(v), C.byref(p), C.c_void_p(self.d_records.ptr),
+             C.c_void_p(self.d_status.ptr), _sp(self.stream))
+
+    def trace_status(self) -> int:
+        return int(self.d_status.download(np.uint32, 1, self.stream)[0])
+
+    def records(self) -> np.ndarray:
+        return self.d_records.download(PHOTON_RECORD, self.n_photon_slots, self.stream)
+
+    def build_table(self, step: float = CELL_STEP) -> dict:
+        self.table.build(self.d_records.ptr, self.n_photon_slots, step, self.sorter, self.stream)
+        return self.table.info()
+
+    def lookup(self, event: int, capacity: int, d_paths: DeviceBuffer | None = None, scene_index: int = 0):
+        """lookupFlatten for one event (0 = all); returns (paths buffer, count).
+        Reads the count back to the host, as the reference does (pk.h:1502-1503)."""
+        if d_paths is None or d_paths.nbytes < capacity * PATH.itemsize:
+            d_paths = DeviceBuffer(max(capacity, 1) * PATH.itemsize)
+        call("palz_lookup_flatten", C.c_void_p(self.table.handle), C.c_uint32(event), C.c_void_p(self.d_points.ptr),
+             C.c_uint32(self.n_points), C.c_uint32(scene_index), C.c_void_p(d_paths.ptr), C.c_uint32(capacity),
+             C.c_void_p(self.d_count.ptr), C.c_void_p(self.sorter.handle), _sp(self.stream))
+        n = int(self.d_count.download(np.uint32, 1, self.stream)[0])
+        return d_paths, n
+
+    def solve_plain(self, K: int, event: int, d_paths: DeviceBuffer, n_paths: int,
+                    d_results: DeviceBuffer | None = None) -> None:
+        d = self.desc()
+        call("palz_solve_specular", int(K), C.c_uint32(event), C.byref(d), C.c_void_p(d_paths.ptr),
+             C.c_uint32(n_paths), C.c_void_p(None), C.c_void_p(d_results.ptr if d_results else None),
+             _sp(self.stream))
+
+    def image(self) -> np.ndarray:
+        return self.d_image.download(np.float32, self.n_points * self.n_lambdas, self.stream).reshape(
+            self.n_lambdas, self.scene.height, self.scene.width)
+
+    def clear_image(self) -> None:
+        self.d_image.zero(self.stream)
+
+
+def download_paths(d_paths: DeviceBuffer, n: int) -> np.ndarray:
+    return d_paths.download(PATH, n)
+
+
+def download_results(d_results: DeviceBuffer, n: int) -> np.ndarray:
+    return d_results.download(PATH_RESULT, n)

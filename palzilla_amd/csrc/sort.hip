@@ -66,8 +66,10 @@ __global__ void __launch_bounds__(256) k_sort_prepare(uint32_t* hist, uint32_t* 
 
 __global__ void __launch_bounds__(256) k_sort_count(const uint64_t* __restrict__ words, uint32_t n,
                                                     int start_bit, int end_bit, int npass,
-                                                    uint32_t* __restrict__ hist) {
+                                                    uint32_t* __restrict__ hist,
+                                                    const uint32_t* __restrict__ n_dev) {
     __shared__ uint32_t s_h[kSortMaxPasses * 256];
+    if (n_dev) n = min(n, *n_dev);
     for (int i = threadIdx.x; i < npass * 256; i += 256) s_h[i] = 0u;
     __syncthreads();
     const uint32_t stride = gridDim.x * 256u;
@@ -92,7 +94,7 @@ __global__ void __launch_bounds__(kSortThreads)
                 const uint32_t* __restrict__ vsrc, uint32_t* __restrict__ vdst, uint32_t n,
                 int shift, uint32_t mask, int pass, const uint32_t* __restrict__ hist,
                 uint64_t* __restrict__ lookback, uint32_t* __restrict__ ticket,
-                const uint32_t* __restrict__ epoch) {
+                const uint32_t* __restrict__ epoch, const uint32_t* __restrict__ n_dev) {
     __shared__ uint32_t s_wave_hist[kSortWaves][256];
     __shared__ int32_t s_local_start[256];
     __shared__ int32_t s_scatter_base[256];
@@ -111,6 +113,10 @@ __global__ void __launch_bounds__(kSortThreads)
     __syncthreads();
 
     const uint32_t tile = s_tile;
+    if (n_dev) {
+        n = min(n, *n_dev);  // blocks past the device-side count have nothing to do
+        if (tile * (uint32_t)kSortTile >= n) return;
+    }
     const uint32_t ep = epoch[0] * 8u + (uint32_t)pass;
     const uint32_t tile_base = tile * (uint32_t)kSortTile;
     const uint32_t wave_base = tile_base + (uint32_t)wave * (64u * kSortItems);
@@ -224,15 +230,15 @@ void launch_sort_prepare(const SortScratch& s, palz_aabb* aabb, hipStream_t stre
 }
 
 void launch_sort_count(const SortScratch& s, const uint64_t* words, uint32_t n, int start_bit,
-                       int end_bit, hipStream_t stream) {
+                       int end_bit, hipStream_t stream, const uint32_t* n_dev) {
     const int npass = (int)div_up((uint32_t)(end_bit - start_bit), 8u);
     const uint32_t blocks = std::max(1u, std::min(div_up(n, 256u * 16u), 2048u));
-    k_sort_count<<<blocks, 256, 0, stream>>>(words, n, start_bit, end_bit, npass, s.hist);
+    k_sort_count<<<blocks, 256, 0, stream>>>(words, n, start_bit, end_bit, npass, s.hist, n_dev);
 }
 
 int launch_sort_passes(const SortScratch& s, uint64_t* words, uint64_t* tmp, uint32_t* values,
                        uint32_t* tmp_values, uint32_t n, int start_bit, int end_bit,
-                       hipStream_t stream, hipEvent_t* pass_events) {
+                       hipStream_t stream, hipEvent_t* pass_events, const uint32_t* n_dev) {
     const int npass = (int)div_up((uint32_t)(end_bit - start_bit), 8u);
     const uint32_t tiles = div_up(n, (uint32_t)kSortTile);
     uint64_t* src = words;
@@ -246,11 +252,11 @@ int launch_sort_passes(const SortScratch& s, uint64_t* words, uint64_t* tmp, uin
         if (values) {
             k_sort_pass<true><<<tiles, kSortThreads, 0, stream>>>(
                 src, dst, vsrc, vdst, n, shift, mask, p, s.hist + p * 256, s.lookback,
-                s.tickets + p, s.epoch);
+                s.tickets + p, s.epoch, n_dev);
         } else {
             k_sort_pass<false><<<tiles, kSortThreads, 0, stream>>>(
                 src, dst, nullptr, nullptr, n, shift, mask, p, s.hist + p * 256, s.lookback,
-                s.tickets + p, s.epoch);
+                s.tickets + p, s.epoch, n_dev);
         }
         if (pass_events) (void)hipEventRecord(pass_events[p], stream);
         std::swap(src, dst);

@@ -27,15 +27,17 @@ struct SortScratch {
 void launch_sort_prepare(const SortScratch& s, palz_aabb* aabb, hipStream_t stream);
 
 // Count the digits of every pass in one read of the words.
+// `n_dev` (optional): device-side count; n is then the capacity.
 void launch_sort_count(const SortScratch& s, const uint64_t* words, uint32_t n, int start_bit,
-                       int end_bit, hipStream_t stream);
+                       int end_bit, hipStream_t stream, const uint32_t* n_dev = nullptr);
 
 // Run the digit passes after the histograms are complete.  Returns the number
 // of passes; when it is odd the sorted words are in `tmp`.
 // `pass_events`, if not null, receives one event record after each pass.
 int launch_sort_passes(const SortScratch& s, uint64_t* words, uint64_t* tmp, uint32_t* values,
                        uint32_t* tmp_values, uint32_t n, int start_bit, int end_bit,
-                       hipStream_t stream, hipEvent_t* pass_events = nullptr);
+                       hipStream_t stream, hipEvent_t* pass_events = nullptr,
+                       const uint32_t* n_dev = nullptr);
 
 }  // namespace palz
 
