@@ -286,6 +286,74 @@ int palz_solve_specular(int K, uint32_t event, const palz_scene_desc* scene, con
                         uint32_t n_paths, const uint32_t* d_count, palz_path_result* d_results,
                         void* stream);
 
+/* ---- batched persistent solver (no reference counterpart) ---------------
+ * Replaces the reference's per-event-descriptor launch loop (pk.h:1480-1586,
+ * one lookupFlatten + blocking count read + solveSpecularPath_K launch per
+ * descriptor, per scene) with one batch over many scenes and chain lengths.
+ * Work items are (path, wavelength) with item = path * n_lambdas + l.  For
+ * each K = 1..3 a persistent kernel refills idle lanes at Newton-iteration
+ * granularity from a chunked work queue; converged items then go through the
+ * throughput / dA/dw stage and, with `refine`, one FP64 Newton step on the
+ * converged solution (PALZ_REFINE_*).  Chains with K = 4 are not
+ * part of a batch (status NOT_CONVERGED, counted in stats[3]).  Every count
+ * stays on the device, so a solve can be captured in a hipGraph. */
+typedef struct palz_batched palz_batched;
+
+/* Launch shape of one chain length: lanes per block, resident blocks per
+ * CU (grid = blocks_per_cu * CU count), items handed out per queue ticket. */
+typedef struct {
+    int32_t block;
+    int32_t blocks_per_cu;
+    int32_t chunk;
+    int32_t finish_block;
+} palz_launch_config;
+
+int palz_batched_create(palz_batched** out);
+int palz_batched_destroy(palz_batched* b);
+/* Stable partition of n_paths paths by chain length into the batch's own
+ * buffer.  `d_count` (optional, device) replaces n_paths, which is then the
+ * capacity.  After the call, palz_batched_order gives, for every batch slot,
+ * the index of the path in d_paths. */
+int palz_batched_prepare(palz_batched* b, const palz_path* d_paths, uint32_t n_paths, const uint32_t* d_count,
+                         palz_sorter* sorter, void* stream);
+int palz_batched_order(palz_batched* b, const uint32_t** d_order, const palz_path** d_sorted_paths,
+                       const uint32_t** d_ranges);
+int palz_batched_set_config(palz_batched* b, int K, const palz_launch_config* cfg);
+int palz_batched_get_config(palz_batched* b, int K, palz_launch_config* cfg);
+/* Solve every prepared item.  d_scenes: device array of n_scenes scene
+ * descriptors (a path's `scene` indexes it); every scene's lambdas array
+ * holds at least n_lambdas entries.  d_results (optional) is indexed by batch
+ * item (batch slot * n_lambdas + l); d_refined (optional) receives 8 doubles
+ * per item: the FP64-refined barycentrics of converged items. */
+enum {
+    PALZ_REFINE_NONE = 0,     /* FP32 Newton only: results equal palz_solve_specular's */
+    PALZ_REFINE_VERTICES = 1, /* + one FP64 Newton step on converged chains; reported
+                                 params are refined, irradiance uses the FP32 chain */
+    PALZ_REFINE_ALL = 2       /* as 1, and the refined chain also feeds dA/dw */
+};
+int palz_batched_solve(palz_batched* b, const palz_scene_desc* d_scenes, uint32_t n_scenes, uint32_t n_lambdas,
+                       int refine, palz_path_result* d_results, double* d_refined, void* stream);
+/* Capture palz_batched_solve with these arguments in a hipGraph (first call
+ * or changed arguments; captured on the batch's own stream) and launch the
+ * graph on `stream`.  palz_batched_prepare and palz_batched_set_config drop
+ * the captured graph. */
+int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, uint32_t n_scenes,
+                             uint32_t n_lambdas, int refine, palz_path_result* d_results, double* d_refined,
+                             void* stream);
+/* Per chain length K = 1..3 (row K-1): [0] items, [1] converged, [2]
+ * contributing, [3] residual evaluations (Newton iterations plus the final
+ * converged check); row 3: [2] items whose path names a scene >= n_scenes
+ * (not solved, status NOT_CONVERGED), [3] items skipped (K = 4).
+ * Synchronises the stream. */
+int palz_batched_stats(palz_batched* b, uint64_t* stats16, void* stream);
+/* Time every candidate launch shape per chain length on the prepared batch
+ * and keep the fastest (the autotuner).  The image receives the candidates'
+ * contributions; clear it afterwards.  Writes the chosen shapes and their
+ * times to `report` (3 rows of {block, blocks_per_cu, chunk, finish_block,
+ * newton_ms * 1000, finish_ms * 1000}) if not NULL. */
+int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint32_t n_scenes, uint32_t n_lambdas,
+                          int refine, int32_t* report, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,7 @@ _SIGS = {
                                       C.c_uint32, _P]),
     "orc_solve_paths": (None, [_P, _P, C.c_uint32, _P, C.c_int]),
     "orc_solve_f64": (C.c_int, [_P, _P, _P, _P, _P, C.c_double, C.c_double, C.c_int, _P, _P]),
+    "orc_count_flops": (C.c_int, [_P, _P, _P, _P, _P, C.c_float, _P]),
     "orc_max_stack_depth": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
 }
 
@@ -214,3 +215,17 @@ def solve_f64(scene, path, p_end, eta: float, params0, tol: float = 1e-28, max_i
     ok = lib().orc_solve_f64(_ptr(scene.tris), _ptr(scene.attribs), _ptr(p), _ptr(pl), _ptr(pe), float(eta), tol,
                              max_iter, _ptr(prm), _ptr(verts))
     return bool(ok), prm[:2 * K], verts[:3 * K].reshape(K, 3)
+
+
+def count_flops(scene, path, p_end, eta: float):
+    """(evaluation, step) FLOPs of one reference Newton iteration on this chain:
+    the 2K single-tangent residual/Jacobian passes with the cost, then the QR
+    step, update and triangle test (counted by a FLOP-counting scalar)."""
+    out = np.zeros(2, dtype=np.uint64)
+    p = np.ascontiguousarray(np.array([path]))
+    pl = np.asarray(scene.p_light, dtype=np.float32)
+    pe = np.asarray(p_end, dtype=np.float32)
+    if not lib().orc_count_flops(_ptr(scene.tris), _ptr(scene.attribs), _ptr(p), _ptr(pl), _ptr(pe), float(eta),
+                                 _ptr(out)):
+        raise ValueError("chain length must be 1..4")
+    return int(out[0]), int(out[1])

@@ -98,6 +98,34 @@ void square2triangle(float* x, float* y) {  // pk.h:165-178
 }
 
 // ---------------------------------------------------------------------------
+// FLOP-counting scalar: every + - * / and sqrt on it counts one operation
+// (sign flips and comparisons count none).  Instantiating the solver with it
+// counts the reference iteration's arithmetic exactly.
+struct Flop {
+    double v;
+    static thread_local uint64_t n;
+    Flop() : v(0) {}
+    Flop(double x) : v(x) {}
+};
+thread_local uint64_t Flop::n = 0;
+inline Flop operator+(Flop a, Flop b) { ++Flop::n; return a.v + b.v; }
+inline Flop operator-(Flop a, Flop b) { ++Flop::n; return a.v - b.v; }
+inline Flop operator*(Flop a, Flop b) { ++Flop::n; return a.v * b.v; }
+inline Flop operator/(Flop a, Flop b) { ++Flop::n; return a.v / b.v; }
+inline Flop operator-(Flop a) { return -a.v; }
+inline Flop& operator+=(Flop& a, Flop b) { return a = a + b; }
+inline Flop& operator-=(Flop& a, Flop b) { return a = a - b; }
+inline Flop& operator*=(Flop& a, Flop b) { return a = a * b; }
+inline bool operator<(Flop a, Flop b) { return a.v < b.v; }
+inline bool operator>(Flop a, Flop b) { return a.v > b.v; }
+template <class T>
+T tsqrt(T x) { return std::sqrt(x); }
+inline Flop tsqrt(Flop x) { ++Flop::n; return std::sqrt(x.v); }
+template <class T>
+T tabs(T x) { return std::fabs(x); }
+inline Flop tabs(Flop x) { return std::fabs(x.v); }
+
+// ---------------------------------------------------------------------------
 // saka::dval (saka.h:11-208): one tangent, generic scalar
 template <class T>
 struct dval {
@@ -131,8 +159,8 @@ dval<T> operator/(dval<T> x, dval<T> y) { return binary(x, y, x.v / y.v, T(1) / 
 template <class T>
 dval<T> sqrt(dval<T> x) {
     dval<T> u;
-    u.v = std::sqrt(x.v);
-    u.g = x.g * (T(0.5) / std::sqrt(x.v));
+    u.v = tsqrt(x.v);
+    u.g = x.g * (T(0.5) / tsqrt(x.v));
     return u;
 }
 template <class T>
@@ -190,9 +218,9 @@ void solve_qr_overdetermined(Mat<T, R, C> A, Mat<T, R, 1> b, T* x) {  // sen.h:6
     for (int i = 0; i < H; i++) {
         for (int j = 0; j < R; j++) vs(j, i) = j < i ? T(0) : -A(j, i);
         T sgn = T(0) < vs(i, i) ? T(1) : T(-1);
-        T x_len = std::sqrt(column_dot(vs, i, i));
+        T x_len = tsqrt(column_dot(vs, i, i));
         vs(i, i) += sgn * x_len;
-        T v_len2 = T(2) * x_len * (x_len + std::fabs(A(i, i)));
+        T v_len2 = T(2) * x_len * (x_len + tabs(A(i, i)));
         A(i, i) = sgn * x_len;
         for (int r = i + 1; r < R; r++) A(r, i) = T(0);
         for (int c = i + 1; c < C; c++) {
@@ -891,6 +919,42 @@ int orc_solve_f64(const palz_triangle* tris_all, const palz_triangle_attrib* att
         vertices_out[3 * k + 2] = x.z;
     }
     return ok ? 1 : 0;
+}
+
+// Arithmetic of one solveConstraints iteration (pk.h:477-590) on this chain,
+// in the reference's form (2K single-tangent passes): out[0] = the residual
+// and Jacobian evaluation with the cost, out[1] = the QR step, the update and
+// the triangle test that follow it.
+int orc_count_flops(const palz_triangle* tris_all, const palz_triangle_attrib* attribs_all, const palz_path* path,
+                    const float* p_light, const float* p_end, float eta, uint64_t* out) {
+    const int K = (int)(path->event >> 24);
+    palz_triangle tris[4];
+    palz_triangle_attrib attribs[4];
+    for (int k = 0; k < K; k++) {
+        tris[k] = tris_all[path->tris[k]];
+        attribs[k] = attribs_all[path->tris[k]];
+    }
+    vec3<Flop> pb = {p_light[0], p_light[1], p_light[2]}, pe = {p_end[0], p_end[1], p_end[2]};
+    uint64_t counts[2];
+    for (int pass = 0; pass < 2; ++pass) {
+        Flop prm[8];
+        for (int i = 0; i < 2 * K; i++) prm[i] = path->params[i];
+        // pass 0: the cost test succeeds (evaluation only); pass 1: it fails
+        const Flop tol = pass == 0 ? Flop(1e300) : Flop(-1.0);
+        int iters = 0;
+        Flop::n = 0;
+        switch (K) {
+            case 1: solveConstraints<1, Flop>(prm, pb, pe, tris, attribs, Flop(eta), path->event, 1, tol, 0, &iters); break;
+            case 2: solveConstraints<2, Flop>(prm, pb, pe, tris, attribs, Flop(eta), path->event, 1, tol, 0, &iters); break;
+            case 3: solveConstraints<3, Flop>(prm, pb, pe, tris, attribs, Flop(eta), path->event, 1, tol, 0, &iters); break;
+            case 4: solveConstraints<4, Flop>(prm, pb, pe, tris, attribs, Flop(eta), path->event, 1, tol, 0, &iters); break;
+            default: return 0;
+        }
+        counts[pass] = Flop::n;
+    }
+    out[0] = counts[0];
+    out[1] = counts[1] - counts[0];
+    return 1;
 }
 
 int orc_max_stack_depth(const palz_triangle* tris, const palz_internal_node* nodes, uint32_t root, const float* ro,

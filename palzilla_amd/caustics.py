@@ -55,6 +55,9 @@ PATH_RESULT = np.dtype([("status", "<u4"), ("iterations", "<i4"), ("contribution
                         ("footprint", "<f4")])
 assert PATH_RESULT.itemsize == 48
 
+PATH_NOT_CONVERGED, PATH_CONVERGED, PATH_CONTRIBUTES = 0, 1, 2   # palz_path_result.status
+REFINE_NONE, REFINE_VERTICES, REFINE_ALL = 0, 1, 2                # palz_batched_solve refine modes
+
 LIGHT_INTENSITY = 16.0   # m_lightIntencity (pk.h:1686)
 RADIANCE_CLAMP = 12.0    # m_radianceClamp (pk.h:1688)
 MIN_THROUGHPUT = 0.05    # m_minThroughput (pk.h:1687)
@@ -192,3 +195,143 @@ def download_paths(d_paths: DeviceBuffer, n: int) -> np.ndarray:
 
 def download_results(d_results: DeviceBuffer, n: int) -> np.ndarray:
     return d_results.download(PATH_RESULT, n)
+
+
+# -- batched solver --------------------------------------------------------------------------------
+
+class LaunchConfig(C.Structure):
+    _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("chunk", C.c_int32),
+                ("finish_block", C.c_int32)]
+
+
+STAT_ROWS = ("items", "converged", "contributing", "iterations")
+
+
+class BatchedSolver:
+    """palz_batched_*: chain-length partition, persistent Newton kernels,
+    finish kernels with the FP64 refinement step, autotuner and hipGraph.
+
+    Results are indexed by (length-sorted path, wavelength); `order()` maps a
+    sorted path back to its index in the list given to `prepare`."""
+
+    def __init__(self, sorter: OnesweepSort, stream=None):
+        h = C.c_void_p()
+        call("palz_batched_create", C.byref(h))
+        self.handle = h.value
+        self.sorter = sorter
+        self.stream = stream
+        self.n_paths = 0
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            try:
+                lib().palz_batched_destroy(C.c_void_p(self.handle))
+            except Exception:
+                pass
+            self.handle = None
+
+    def prepare(self, d_paths: DeviceBuffer | int, n_paths: int, d_count: int | None = None) -> None:
+        ptr = d_paths.ptr if isinstance(d_paths, DeviceBuffer) else d_paths
+        call("palz_batched_prepare", C.c_void_p(self.handle), C.c_void_p(ptr), C.c_uint32(n_paths),
+             C.c_void_p(d_count), C.c_void_p(self.sorter.handle), _sp(self.stream))
+        self.n_paths = n_paths
+
+    def pointers(self) -> tuple[int, int, int]:
+        o, p, r = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        call("palz_batched_order", C.c_void_p(self.handle), C.byref(o), C.byref(p), C.byref(r))
+        return o.value, p.value, r.value
+
+    def order(self) -> np.ndarray:
+        o, _, _ = self.pointers()
+        return download_ptr(o, np.uint32, self.ranges()[5], self.stream)
+
+    def sorted_paths(self) -> np.ndarray:
+        _, p, _ = self.pointers()
+        return download_ptr(p, PATH, self.ranges()[5], self.stream)
+
+    def ranges(self) -> np.ndarray:
+        """[count with chain length <= k for k = 0..4, path count]."""
+        _, _, r = self.pointers()
+        return download_ptr(r, np.uint32, 6, self.stream)
+
+    def set_config(self, K: int, block: int, blocks_per_cu: int, chunk: int, finish_block: int) -> None:
+        c = LaunchConfig(block, blocks_per_cu, chunk, finish_block)
+        call("palz_batched_set_config", C.c_void_p(self.handle), int(K), C.byref(c))
+
+    def get_config(self, K: int) -> dict:
+        c = LaunchConfig()
+        call("palz_batched_get_config", C.c_void_p(self.handle), int(K), C.byref(c))
+        return {f: getattr(c, f) for f, _ in LaunchConfig._fields_}
+
+    def solve(self, d_scenes: DeviceBuffer, n_scenes: int, n_lambdas: int, refine: int = 1,
+              d_results: DeviceBuffer | None = None, d_refined: DeviceBuffer | None = None,
+              graph: bool = False) -> None:
+        name = "palz_batched_solve_graph" if graph else "palz_batched_solve"
+        call(name, C.c_void_p(self.handle), C.c_void_p(d_scenes.ptr), C.c_uint32(n_scenes), C.c_uint32(n_lambdas),
+             int(refine), C.c_void_p(d_results.ptr if d_results else None),
+             C.c_void_p(d_refined.ptr if d_refined else None), _sp(self.stream))
+
+    def stats(self) -> dict:
+        s = (C.c_uint64 * 16)()
+        call("palz_batched_stats", C.c_void_p(self.handle), s, _sp(self.stream))
+        a = np.frombuffer(s, dtype=np.uint64).copy()
+        out = {f"K{k + 1}": dict(zip(STAT_ROWS, (int(v) for v in a[4 * k:4 * k + 4]))) for k in range(3)}
+        out["skipped"] = int(a[15])
+        return out
+
+    def autotune(self, d_scenes: DeviceBuffer, n_scenes: int, n_lambdas: int, refine: int = 1) -> list[dict]:
+        rep = (C.c_int32 * 18)()
+        call("palz_batched_autotune", C.c_void_p(self.handle), C.c_void_p(d_scenes.ptr), C.c_uint32(n_scenes),
+             C.c_uint32(n_lambdas), int(refine), rep, _sp(self.stream))
+        keys = ("block", "blocks_per_cu", "chunk", "finish_block", "newton_us", "finish_us")
+        return [dict(zip(keys, (int(v) for v in rep[6 * k:6 * k + 6]))) for k in range(3)]
+
+
+def upload_scene_descs(descs: list[SceneDesc], stream=None) -> DeviceBuffer:
+    arr = (SceneDesc * len(descs))(*descs)
+    raw = np.frombuffer(bytes(arr), dtype=np.uint8)
+    return DeviceBuffer.from_array(raw, stream)
+
+
+class SceneBatch:
+    """Many scenes in one path list: each scene is traced, its tuple table
+    built and looked up over all its receiver points (scene index = position
+    in the batch), and the path lists are concatenated on the device."""
+
+    def __init__(self, scene_list: list[CausticScene], sorter: OnesweepSort | None = None, stream=None):
+        self.sorter = sorter or OnesweepSort()
+        self.stream = stream
+        self.scenes = [GpuScene(s, self.sorter, stream) for s in scene_list]
+        L = {g.n_lambdas for g in self.scenes}
+        if len(L) != 1:
+            raise ValueError("all scenes of a batch need the same number of wavelengths")
+        self.n_lambdas = L.pop()
+        self.counts: list[int] = []
+        self.offsets: list[int] = []
+        self.d_paths: DeviceBuffer | None = None
+        self.n_paths = 0
+        self.d_scenes: DeviceBuffer | None = None
+
+    def build(self, iteration: int = 0, capacity_per_scene: int = 1 << 22) -> int:
+        lists = []
+        for i, g in enumerate(self.scenes):
+            g.trace(iteration)
+            g.build_table()
+            d, n = g.lookup(0, capacity=capacity_per_scene, scene_index=i)
+            if n > capacity_per_scene:
+                d, n = g.lookup(0, capacity=n, scene_index=i)
+            lists.append((d, n))
+        self.counts = [n for _, n in lists]
+        self.offsets = list(np.cumsum([0] + self.counts[:-1]))
+        self.n_paths = int(sum(self.counts))
+        self.d_paths = DeviceBuffer(max(self.n_paths, 1) * PATH.itemsize)
+        for (d, n), off in zip(lists, self.offsets):
+            if n:
+                call("palz_memcpy_d2d", C.c_void_p(self.d_paths.ptr + int(off) * PATH.itemsize), C.c_void_p(d.ptr),
+                     C.c_size_t(n * PATH.itemsize), _sp(self.stream))
+        self.d_scenes = upload_scene_descs([g.desc() for g in self.scenes], self.stream)
+        return self.n_paths
+
+    def clear_images(self) -> None:
+        for g in self.scenes:
+            g.clear_image()

@@ -38,6 +38,8 @@ PALZ_HD d3 mkd(v3 a) { return {(double)a.x, (double)a.y, (double)a.z}; }
 PALZ_HD d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 PALZ_HD d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 
+PALZ_HD d3 operator*(d3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+
 template <class F>
 struct vec3_of;
 template <>
@@ -48,6 +50,12 @@ template <>
 struct vec3_of<double> {
     using type = d3;
 };
+template <class F>
+PALZ_HD typename vec3_of<F>::type to_w(v3 a);
+template <>
+PALZ_HD v3 to_w<float>(v3 a) { return a; }
+template <>
+PALZ_HD d3 to_w<double>(v3 a) { return mkd(a); }
 
 // ---------------------------------------------------------------------------
 // Forward-mode dual with NT tangents.  Each tangent is computed with exactly
@@ -275,11 +283,14 @@ PALZ_HD void chain_residual(const F* params, int first, V p_beg, V p_end, const 
             pu.g[t] = (2 * k + 0 == first + t) ? F(1) : F(0);
             pv.g[t] = (2 * k + 1 == first + t) ? F(1) : F(0);
         }
-        // e0 = vs[1] - vs[0], e1 = vs[2] - vs[0] in float (pk.h:501-502, :506-507)
-        const v3 e0 = tri[k].v1 - tri[k].v0, e1 = tri[k].v2 - tri[k].v0;
-        const v3 ne0 = tri[k].s1 - tri[k].s0, ne1 = tri[k].s2 - tri[k].s0;
-        vert[k + 1] = dc3<NT, F>(tri[k].v0) + dc3<NT, F>(e0) * pu + dc3<NT, F>(e1) * pv;
-        nrm[k] = dc3<NT, F>(tri[k].s0) + dc3<NT, F>(ne0) * pu + dc3<NT, F>(ne1) * pv;
+        // e0 = vs[1] - vs[0], e1 = vs[2] - vs[0] (pk.h:501-502, :506-507) in the
+        // working precision: float ops for F = float, exact for F = double
+        using W = typename vec3_of<F>::type;
+        const W v0 = to_w<F>(tri[k].v0), s0 = to_w<F>(tri[k].s0);
+        const W e0 = to_w<F>(tri[k].v1) - v0, e1 = to_w<F>(tri[k].v2) - v0;
+        const W ne0 = to_w<F>(tri[k].s1) - s0, ne1 = to_w<F>(tri[k].s2) - s0;
+        vert[k + 1] = dc3<NT, F>(v0) + dc3<NT, F>(e0) * pu + dc3<NT, F>(e1) * pv;
+        nrm[k] = dc3<NT, F>(s0) + dc3<NT, F>(ne0) * pu + dc3<NT, F>(ne1) * pv;
     }
     F c_acc = F(0);
 #pragma unroll
