@@ -1,18 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X caustic hot path; prints ONE JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lbvh]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c4|lbvh]
 
-Workload `lbvh`: the LBVH + onesweep build of 16,777,216 triangles (centroids
-U[0,1]^3, lognormal sizes, random orientation; 30-bit Morton keys).  A step is
-one full build (scene AABB, Morton keys + digit counts, four sort passes,
-deltas, bottom-up build) over triangles already resident in HBM.
+Default workload `c2`, the headline: spectral two-interface refraction through
+a glass slab (256x256-cell wavy top, 131,076 triangles, 2^22 photons, 1024^2
+receiver, 16 wavelengths, Cauchy n = 1.5 + 4200 nm^2 / lambda^2).  Photons are
+traced, the tuple table built and the (receiver point, tuple) list flattened
+once; a step is then one batched solve of every (path, wavelength) item with
+chain length 1..3: persistent FP32 Newton kernels, then throughput, dA/dw and
+irradiance with one FP64 refinement step on every converged chain, per-item
+results written to HBM.  The value is converged, contributing specular paths
+per second (scaled by the fraction of a sample whose refined vertices lie
+within 1e-5 of the FP64 oracle).  Beside it: the plain per-descriptor solver
+(one launch per event descriptor, as the reference renders) on the same tuple
+set, the CPU oracle solve of a bounded sample, and the sub-results `c4` (256
+scenes in one hipGraph launch vs per-scene plain launches) and `lbvh` (16M
+triangle LBVH + onesweep build) unless --no-extras.
 
 Only one GPU is used: --gpus above 1 exits with a message.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -21,8 +32,12 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+import numpy as np  # noqa: E402
+
 METRIC = "converged specular paths/s at 1e-5 vertex err, batched vs plain; LBVH+sort Mtri/s"
 HBM_PEAK_GBPS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+CPU_THREADS = 16  # the GPU box's host-core share for one GPU
 
 # Algorithmic bytes per triangle of each LBVH stage (the reference's pass
 # structure): AABB read 36; Morton read 36 + write 8; sort pass read 8 + write
@@ -42,7 +57,6 @@ def log(msg: str) -> None:
 def measure_copy_bandwidth(nbytes: int = 1 << 30, reps: int = 5) -> float:
     from palzilla_amd._lib import call
     from palzilla_amd.device import DeviceBuffer, Event, Stream
-    import ctypes as C
     src, dst = DeviceBuffer(nbytes), DeviceBuffer(nbytes)
     src.zero()
     st = Stream()
@@ -60,39 +74,379 @@ def measure_copy_bandwidth(nbytes: int = 1 << 30, reps: int = 5) -> float:
     return 2.0 * nbytes / (best * 1e-3) / 1e9
 
 
-def bench_lbvh(args) -> dict:
-    import numpy as np
+# ---------------------------------------------------------------------------------------------------
+# spectral slab (headline)
 
+def _solver_flops(stats: dict, event_share: dict) -> dict:
+    """Algorithmic FLOPs of the Newton kernels per chain length: residual
+    evaluations and steps times the reference iteration's counts, weighted by
+    the event mix of that chain length."""
+    from palzilla_amd import caustics as CA
+    out = {}
+    for K in (1, 2, 3):
+        row = stats[f"K{K}"]
+        share = {ev: w for ev, w in event_share.items() if ev >> 24 == K}
+        tot = sum(share.values())
+        if not tot:
+            out[K] = 0.0
+            continue
+        f_eval = sum(CA.iteration_flops(ev)[0] * w for ev, w in share.items()) / tot
+        f_step = CA.ITERATION_FLOPS_STEP[K]
+        steps = row["iterations"] - row["converged"]
+        out[K] = row["iterations"] * f_eval + steps * f_step
+    return out
+
+
+def bench_c2(args) -> dict:
+    from palzilla_amd import caustics as CA
+    from palzilla_amd import scenes
+    from palzilla_amd.device import DeviceBuffer, Event, Stream
+
+    st = Stream()
+    t_setup = time.perf_counter()
+    s = scenes.config2()
+    L = s.lambdas.shape[0]
+    log(f"[bench] c2: {s.n_tris} triangles, {s.n_tris * s.photons_per_triangle} photons, "
+        f"{s.points.shape[0]} receiver points, {L} wavelengths")
+    g = CA.GpuScene(s, stream=st)
+    e = [Event() for _ in range(6)]
+    e[0].record(st)
+    g.trace()
+    e[1].record(st)
+    info = g.build_table()
+    e[2].record(st)
+    _, n_all = g.lookup(0, capacity=1)
+    d_all, n_all = g.lookup(0, capacity=n_all)
+    e[3].record(st)
+    st.sync()
+    recs_event = g.records()["event"]
+    events = sorted(int(v) for v in np.unique(recs_event[recs_event != 0]))
+    log(f"[bench] c2: {info['n_tuples']} tuples, {n_all} paths")
+
+    # plain: one list and one launch per event descriptor (pk.h:1480-1586)
+    plain_lists = []
+    for ev in events:
+        K = ev >> 24
+        if K > 3:
+            continue
+        _, n = g.lookup(ev, capacity=1)
+        if n:
+            d, n = g.lookup(ev, capacity=n)
+            plain_lists.append((K, ev, d, n))
+    n_plain_paths = sum(x[3] for x in plain_lists)
+
+    bs = CA.BatchedSolver(g.sorter, stream=st)
+    e[4].record(st)
+    bs.prepare(d_all, n_all)
+    e[5].record(st)
+    st.sync()
+    ranges = bs.ranges()
+    assert ranges[3] == n_plain_paths, "batched and plain solve different tuple sets"
+    pipeline_ms = {"photon_trace": e[0].elapsed_ms(e[1]), "tuple_table": e[1].elapsed_ms(e[2]),
+                   "lookup_flatten": e[2].elapsed_ms(e[3]), "batched_prepare": e[4].elapsed_ms(e[5])}
+    n_items = int(ranges[3]) * L
+    d_scenes = CA.upload_scene_descs([g.desc()], st)
+    d_res = DeviceBuffer(max(int(ranges[5]) * L, 1) * CA.PATH_RESULT.itemsize)
+
+    if args.launch_config:
+        for K, c in enumerate(json.loads(args.launch_config), start=1):
+            bs.set_config(K, *c)
+        tune = [bs.get_config(K) for K in (1, 2, 3)]
+        log(f"[bench] c2: launch shapes {tune}")
+    else:
+        log("[bench] c2: autotune")
+        tune = bs.autotune(d_scenes, 1, L, refine=CA.REFINE_VERTICES)
+        log(f"[bench] c2: tuned {tune}")
+
+    # batched, timed region: eager launches with per-kernel events on the stream
+    for _ in range(args.warmup):
+        bs.solve(d_scenes, 1, L, CA.REFINE_VERTICES, d_res)
+    bs.set_timing(True)
+    kern = {k: 0.0 for k in CA.BatchedSolver.TIMED_KERNELS}
+    st.sync()
+    t0, t1 = Event(), Event()
+    log(f"[bench] c2: timing {args.steps} batched steps")
+    wall0 = time.perf_counter()
+    t0.record(st)
+    for _ in range(args.steps):
+        bs.solve(d_scenes, 1, L, CA.REFINE_VERTICES, d_res)
+        for k, v in bs.timings().items():
+            kern[k] += v / args.steps
+    t1.record(st)
+    st.sync()
+    wall = time.perf_counter() - wall0
+    bs.set_timing(False)
+    ms = t0.elapsed_ms(t1) / args.steps
+    stats = bs.stats()
+
+    # the same solve as one hipGraph launch
+    g.clear_image()
+    bs.solve(d_scenes, 1, L, CA.REFINE_VERTICES, d_res, graph=True)
+    st.sync()
+    t0.record(st)
+    for _ in range(args.steps):
+        bs.solve(d_scenes, 1, L, CA.REFINE_VERTICES, d_res, graph=True)
+    t1.record(st)
+    st.sync()
+    ms_graph = t0.elapsed_ms(t1) / args.steps
+    g.clear_image()
+    bs.solve(d_scenes, 1, L, CA.REFINE_VERTICES, d_res)
+    img_batched = g.image().reshape(-1).astype(np.float64)
+
+    # plain, same tuple set, image only (as the reference renders)
+    log(f"[bench] c2: timing {args.steps} plain steps ({len(plain_lists)} launches each)")
+    for _ in range(max(1, args.warmup)):
+        for K, ev, d, n in plain_lists:
+            g.solve_plain(K, ev, d, n)
+    pe = [(Event(), Event()) for _ in plain_lists]
+    plain_kernel_ms = {}
+    g.clear_image()
+    st.sync()
+    t0.record(st)
+    for _ in range(args.steps):
+        for (K, ev, d, n), (a, b) in zip(plain_lists, pe):
+            a.record(st)
+            g.solve_plain(K, ev, d, n)
+            b.record(st)
+        st.sync()
+        for (K, ev, d, n), (a, b) in zip(plain_lists, pe):
+            plain_kernel_ms[CA.event_str(ev)] = plain_kernel_ms.get(CA.event_str(ev), 0.0) + \
+                a.elapsed_ms(b) / args.steps
+    t1.record(st)
+    st.sync()
+    ms_plain = t0.elapsed_ms(t1) / args.steps
+    g.clear_image()
+    for K, ev, d, n in plain_lists:
+        g.solve_plain(K, ev, d, n)
+    img_plain = g.image().reshape(-1).astype(np.float64)
+    st.sync()
+    img_rmse = float(np.sqrt(np.mean((img_batched - img_plain) ** 2)) /
+                     max(np.sqrt(np.mean(img_plain ** 2)), 1e-30))
+
+    converged = sum(stats[f"K{k}"]["converged"] for k in (1, 2, 3))
+    contributing = sum(stats[f"K{k}"]["contributing"] for k in (1, 2, 3))
+
+    # sample: CPU oracle (baseline + FP32 parity) and FP64 vertex error of the refined chains
+    sample = verify_sample(args, s, g, bs, d_res, n_all, L, st)
+    frac_1e5 = sample["vertex"]["frac_within_1e-5"]
+    value = contributing * frac_1e5 / (ms * 1e-3)
+    # the plain kernel's chains are the FP32 ones (bit-identical to the oracle's)
+    plain_value = contributing * sample["vertex"]["fp32_frac_within_1e-5"] / (ms_plain * 1e-3)
+
+    # roofline of the dominant kernel (the solver is VALU-bound, FP32, no MFMA)
+    event_share = {}
+    for K, ev, d, n in plain_lists:
+        event_share[ev] = event_share.get(ev, 0) + n
+    flops = _solver_flops(stats, event_share)
+    dominant = max(kern, key=kern.get)
+    achieved = None
+    if dominant.startswith("newton"):
+        K = int(dominant[-1])
+        achieved = flops[K] / (kern[dominant] * 1e-3) / 1e12
+    label = {"compact": "k_skipped + exclusive scan + k_compact_scatter"}.get(
+        dominant, f"k_newton<{dominant[-1]}>" if dominant.startswith("newton") else
+        f"k_finish<{dominant[-1]}> + k_refine<{dominant[-1]}>")
+    roof = {"bound": "valu", "kernel": label,
+            "achieved": round(achieved, 2) if achieved is not None else None, "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4) if achieved is not None else None,
+            "traffic": None,
+            "flops_per_launch": flops.get(int(dominant[-1])) if dominant.startswith("newton") else None,
+            "kernel_ms": round(kern[dominant], 3)}
+    setup_s = time.perf_counter() - t_setup
+    return {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "paths/s",
+        "ms_per_step": round(ms, 3),
+        "dtype": "f32+f64",
+        "config": {"workload": "spectral slab: 256x256-cell wavy glass top + flat bottom (131,076 tris), "
+                               "2^22 photons, 1024^2 receiver, 16 wavelengths 380-720 nm, TT + all other "
+                               "chains of length 1..3; batched FP32 Newton + one FP64 refinement step",
+                   "n_triangles": s.n_tris, "photons": s.n_tris * s.photons_per_triangle,
+                   "receiver": [s.width, s.height], "wavelengths": L, "paths": int(ranges[3]),
+                   "items": n_items, "refine": "fp64 step on converged vertices",
+                   "counted": "contributing chains (cost < 1e-10, throughput > 0) x fraction of the "
+                              "sampled chains within 1e-5 relative vertex error of the FP64 oracle"},
+        "roofline": roof,
+        "cpu_baseline": sample["cpu_baseline"],
+        "batched": {"ms_per_step": round(ms, 3), "ms_per_step_graph": round(ms_graph, 3),
+                    "kernel_ms": {k: round(v, 3) for k, v in kern.items()},
+                    "converged_items": converged, "contributing_items": contributing,
+                    "stats": stats, "launch_configs": tune, "wall_s": round(wall, 3),
+                    "tflops_by_K": {K: round(flops[K] / (kern[f"newton_k{K}"] * 1e-3) / 1e12, 2)
+                                    for K in (1, 2, 3) if kern[f"newton_k{K}"] > 0}},
+        "plain": {"value": round(plain_value, 1), "unit": "paths/s", "ms_per_step": round(ms_plain, 3),
+                  "launches": len(plain_lists), "kernel_ms": {k: round(v, 3) for k, v in plain_kernel_ms.items()},
+                  "batched_over_plain": round(value / plain_value, 2) if plain_value else None,
+                  "batched_over_plain_time": round(ms_plain / ms, 2),
+                  "image_rel_rmse_batched_vs_plain": img_rmse},
+        "vertex_error": sample["vertex"],
+        "parity_sample": sample["parity"],
+        "pipeline_ms": {k: round(v, 3) for k, v in pipeline_ms.items()},
+        "setup_s": round(setup_s, 1),
+    }
+
+
+def verify_sample(args, s, g, bs, d_res, n_all, L, st) -> dict:
+    """A bounded, evenly spread sample of the path list: the CPU oracle solves
+    it (the CPU baseline; its FP32 results must equal the GPU's), and the FP64
+    oracle measures the vertex error of the GPU's refined chains."""
+    from palzilla_amd import caustics as CA
+    import oracle
+
+    n_chunks = 64
+    chunk = max(1, min(args.cpu_paths // n_chunks, n_all // n_chunks))
+    starts = np.linspace(0, n_all - chunk, n_chunks).astype(np.int64)
+    _, p_sorted, _ = bs.pointers()
+    paths, res = [], []
+    for a in starts:
+        a = int(a)
+        paths.append(CA.download_paths_at(p_sorted, a, chunk))
+        res.append(d_res.download(CA.PATH_RESULT, chunk * L, offset_bytes=a * L * CA.PATH_RESULT.itemsize))
+    paths = np.concatenate(paths)
+    res = np.concatenate(res)
+    keep = (paths["event"] >> 24) <= 3
+    paths, res = paths[keep], res[np.repeat(keep, L)]
+    log(f"[bench] cpu baseline: oracle solve of {paths.shape[0]} paths x {L} wavelengths, {CPU_THREADS} threads")
+    nodes = g.bvh.download_nodes()
+    root = g.bvh.download_root()
+    hs = oracle.HostScene(s, nodes, root)
+    t0 = time.perf_counter()
+    want = oracle.solve_paths(hs, paths, threads=CPU_THREADS)
+    dt = time.perf_counter() - t0
+    cpu_contrib = int((want["status"] == CA.PATH_CONTRIBUTES).sum())
+    # FP32 parity of the sample: decisions and irradiance equal; params are refined on the GPU
+    same = {f: bool(np.array_equal(res[f], want[f])) for f in ("status", "iterations", "contribution",
+                                                                 "footprint")}
+    conv = np.nonzero(want["status"] >= 1)[0]
+    rng = np.random.default_rng(7)
+    pick = conv if conv.size <= args.vertex_sample else rng.choice(conv, args.vertex_sample, replace=False)
+    e32, e64 = [], []
+    for it in pick:
+        pi, l = divmod(int(it), L)
+        path = paths[pi]
+        K = int(path["event"]) >> 24
+        lam = np.float32(s.lambdas[l])
+        eta = float(np.float32(s.cauchy_a) + np.float32(s.cauchy_b) / (lam * lam))
+        p_end = s.points[path["pixel"]]["p"]
+        ok, _, x64 = oracle.solve_f64(s, path, p_end, eta, want["params"][it][:2 * K].astype(np.float64),
+                                      tol=1e-28, max_iter=64)
+        v = s.tris[path["tris"][:K]].astype(np.float64).reshape(K, 3, 3)
+
+        def verts(prm):
+            p = np.asarray(prm, dtype=np.float64)[:2 * K].reshape(K, 2)
+            return v[:, 0] + (v[:, 1] - v[:, 0]) * p[:, :1] + (v[:, 2] - v[:, 0]) * p[:, 1:]
+        sc = np.linalg.norm(x64, axis=1)
+        e32.append(np.max(np.linalg.norm(verts(want["params"][it]) - x64, axis=1) / sc))
+        e64.append(np.max(np.linalg.norm(verts(res["params"][it]) - x64, axis=1) / sc))
+    e32, e64 = np.asarray(e32), np.asarray(e64)
+    return {
+        "cpu_baseline": {"value": round(cpu_contrib / dt, 1), "unit": "paths/s", "cores": CPU_THREADS,
+                         "kind": "port",
+                         "sample": f"oracle FP32 solve (solveConstraints + throughput + dA/dw) of "
+                                   f"{paths.shape[0]} paths x {L} wavelengths in {n_chunks} evenly spaced "
+                                   f"chunks of the path list, {CPU_THREADS} threads, {dt:.1f} s"},
+        "parity": {"items": int(res.shape[0]), "equal_fields": same,
+                   "contributing": cpu_contrib},
+        "vertex": {"chains": int(pick.size), "fp32_max": float(e32.max()) if e32.size else None,
+                   "fp32_p99": float(np.quantile(e32, 0.99)) if e32.size else None,
+                   "refined_max": float(e64.max()) if e64.size else None,
+                   "refined_p99": float(np.quantile(e64, 0.99)) if e64.size else None,
+                   "frac_within_1e-5": float(np.mean(e64 <= 1e-5)) if e64.size else 0.0,
+                   "fp32_frac_within_1e-5": float(np.mean(e32 <= 1e-5)) if e32.size else 0.0},
+    }
+
+
+# ---------------------------------------------------------------------------------------------------
+# 256 scenes, batched vs per-scene plain
+
+def bench_c4(args) -> dict:
+    from palzilla_amd import caustics as CA
+    from palzilla_amd import scenes
+    from palzilla_amd.device import DeviceBuffer, Event, Stream
+
+    st = Stream()
+    scene_list = scenes.config4_scenes(n_scenes=args.c4_scenes)
+    log(f"[bench] c4: {len(scene_list)} scenes, {sum(x.n_tris for x in scene_list)} triangles")
+    batch = CA.SceneBatch(scene_list, stream=st)
+    n = batch.build()
+    L = batch.n_lambdas
+    st.sync()
+    # per-scene, per-descriptor lists for the plain launches
+    plain = []
+    for i, gs in enumerate(batch.scenes):
+        ev_all = gs.records()["event"]
+        for ev in sorted(int(v) for v in np.unique(ev_all[ev_all != 0])):
+            if ev >> 24 > 3:
+                continue
+            _, m = gs.lookup(ev, capacity=1, scene_index=i)
+            if m:
+                d, m = gs.lookup(ev, capacity=m, scene_index=i)
+                plain.append((gs, ev >> 24, ev, d, m))
+    bs = CA.BatchedSolver(batch.sorter, stream=st)
+    bs.prepare(batch.d_paths, n)
+    ranges = bs.ranges()
+    assert ranges[3] == sum(x[4] for x in plain)
+    tune = bs.autotune(batch.d_scenes, len(scene_list), L, refine=CA.REFINE_VERTICES)
+    d_res = DeviceBuffer(max(n * L, 1) * CA.PATH_RESULT.itemsize)
+    t0, t1 = Event(), Event()
+    for _ in range(args.warmup + 1):
+        bs.solve(batch.d_scenes, len(scene_list), L, CA.REFINE_VERTICES, d_res, graph=True)
+    st.sync()
+    t0.record(st)
+    for _ in range(args.steps):
+        bs.solve(batch.d_scenes, len(scene_list), L, CA.REFINE_VERTICES, d_res, graph=True)
+    t1.record(st)
+    st.sync()
+    ms = t0.elapsed_ms(t1) / args.steps
+    stats = bs.stats()
+    contributing = sum(stats[f"K{k}"]["contributing"] for k in (1, 2, 3))
+    for _ in range(max(1, args.warmup)):
+        for gs, K, ev, d, m in plain:
+            gs.solve_plain(K, ev, d, m)
+    st.sync()
+    t0.record(st)
+    for _ in range(args.steps):
+        for gs, K, ev, d, m in plain:
+            gs.solve_plain(K, ev, d, m)
+    t1.record(st)
+    st.sync()
+    ms_plain = t0.elapsed_ms(t1) / args.steps
+    return {"workload": "256 stack scenes, 16^2..256^2 cells, K = 1..3, IOR U[1.3,1.8], 2^14..2^20 photons",
+            "paths": int(ranges[3]), "items": int(ranges[3]) * L, "contributing_items": contributing,
+            "batched_graph_ms": round(ms, 3), "plain_ms": round(ms_plain, 3), "plain_launches": len(plain),
+            "batched_paths_per_s": round(contributing / (ms * 1e-3), 1),
+            "plain_paths_per_s": round(contributing / (ms_plain * 1e-3), 1),
+            "batched_over_plain": round(ms_plain / ms, 2), "launch_configs": tune}
+
+
+# ---------------------------------------------------------------------------------------------------
+# LBVH + onesweep
+
+def bench_lbvh(args) -> dict:
     from palzilla_amd import scenes
     from palzilla_amd.device import DeviceBuffer, Event, Stream, sync
     from palzilla_amd.lbvh import BVHGPUBuilder, OnesweepSort
 
     n = args.n
-    log(f"[bench] generating {n} triangles")
+    log(f"[bench] lbvh: generating {n} triangles")
     tris = scenes.triangle_soup(n, seed=2024)
     d_tris = DeviceBuffer.from_array(tris)
     st = Stream()
     sorter = OnesweepSort()
     sorter.reserve(n)
     b = BVHGPUBuilder()
-    log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
         b.build(d_tris.ptr, n, 0, sorter, stream=st, morton_bits=30)
     st.sync()
     sync()
     e0, e1 = Event(), Event()
-    log(f"[bench] timing {args.steps} steps")
-    t0 = time.perf_counter()
     e0.record(st)
     for _ in range(args.steps):
         b.build(d_tris.ptr, n, 0, sorter, stream=st, morton_bits=30)
     e1.record(st)
     st.sync()
-    sync()
-    wall = time.perf_counter() - t0
     ms = e0.elapsed_ms(e1) / args.steps
-
-    # per-kernel durations, live, on the launch stream
     b.set_timing(True)
     stage_ms = {k: 0.0 for k in b.STAGES}
     for _ in range(args.steps):
@@ -100,48 +454,22 @@ def bench_lbvh(args) -> dict:
         for k, v in b.timings().items():
             stage_ms[k] += v / args.steps
     b.set_timing(False)
-    sort_pass_ms = sum(stage_ms[f"sort_pass{i}"] for i in range(4)) / 4.0
-    kernel_total = {"k_sort_pass": 4 * sort_pass_ms, "k_build": stage_ms["build"],
-                    "k_morton_hist": stage_ms["morton_hist"], "k_scene_aabb": stage_ms["scene_aabb"],
-                    "k_deltas": stage_ms["deltas"]}
-    dominant = max(kernel_total, key=kernel_total.get)
-    per_launch_ms = {"k_sort_pass": sort_pass_ms, "k_build": stage_ms["build"],
-                     "k_morton_hist": stage_ms["morton_hist"], "k_scene_aabb": stage_ms["scene_aabb"],
-                     "k_deltas": stage_ms["deltas"]}[dominant]
-    bytes_per_tri = {"k_sort_pass": 16.0, "k_build": STAGE_BYTES_PER_TRI["build"],
-                     "k_morton_hist": 44.0, "k_scene_aabb": 36.0, "k_deltas": 9.0}[dominant]
-    achieved = bytes_per_tri * n / (per_launch_ms * 1e-3) / 1e9
-
-    # parity spot check of the timed configuration: sortedness and leaf count
-    words, deltas = b.download_sorted()
+    words, _ = b.download_sorted()
     keys = (words & np.uint64(0xFFFFFFFF)).astype(np.int64)
     assert np.all(np.diff(keys) >= 0), "sorted keys are not sorted"
     assert np.array_equal(np.sort(words >> np.uint64(32)), np.arange(n, dtype=np.uint64))
-
-    copy_gbps = measure_copy_bandwidth()
-    mtri = n / (ms * 1e-3) / 1e6
-    out = {
-        "metric": METRIC,
-        "value": round(mtri, 2),
-        "unit": "Mtri/s",
-        "ms_per_step": round(ms, 4),
-        "dtype": "u32",
-        "config": {"workload": "lbvh+onesweep, 16M-triangle soup (30-bit Morton keys)",
-                   "n_triangles": n, "morton_bits": 30},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": None},
-        "lbvh": {"stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
-                 "algorithmic_bytes_per_tri": LBVH_BYTES_PER_TRI,
-                 "achieved_GBps": round(LBVH_BYTES_PER_TRI * n / (ms * 1e-3) / 1e9, 1),
-                 "hbm_frac": round(LBVH_BYTES_PER_TRI * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-                 "copy_kernel_GBps": round(copy_gbps, 1)},
-        "wall_s": round(wall, 3),
-    }
+    achieved = LBVH_BYTES_PER_TRI * n / (ms * 1e-3) / 1e9
+    out = {"workload": "lbvh+onesweep, 16M-triangle soup (30-bit Morton keys)", "n_triangles": n,
+           "value": round(n / (ms * 1e-3) / 1e6, 2), "unit": "Mtri/s", "ms_per_step": round(ms, 4),
+           "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+           "algorithmic_bytes_per_tri": LBVH_BYTES_PER_TRI, "achieved_GBps": round(achieved, 1),
+           "hbm_frac": round(achieved / HBM_PEAK_GBPS, 4),
+           "stage_GBps": {k: round(STAGE_BYTES_PER_TRI[k] * n / (v * 1e-3) / 1e9, 1)
+                          for k, v in stage_ms.items() if v > 0},
+           "copy_kernel_GBps": round(measure_copy_bandwidth(), 1)}
     if not args.no_cpu:
         import oracle
         sample = min(n, args.cpu_sample)
-        log(f"[bench] cpu baseline on {sample} triangles, 1 thread")
         t0 = time.perf_counter()
         oracle.lbvh(tris[:sample], bits=30)
         dt = time.perf_counter() - t0
@@ -155,18 +483,40 @@ def bench_lbvh(args) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="lbvh", choices=["lbvh"])
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "lbvh"])
+    ap.add_argument("--no-extras", action="store_true", help="c2 only, without the c4 and lbvh sub-results")
     ap.add_argument("--n", type=int, default=1 << 24)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 24)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 22, help="LBVH CPU baseline triangles")
+    ap.add_argument("--cpu-paths", type=int, default=1 << 21, help="c2 CPU baseline paths")
+    ap.add_argument("--vertex-sample", type=int, default=512)
+    ap.add_argument("--c4-scenes", type=int, default=256)
+    ap.add_argument("--launch-config", default=None,
+                    help="skip the c2 autotuner: JSON [[block, blocks_per_cu, chunk, finish_block]] for K = 1..3")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); nothing is run across GPUs.",
               file=sys.stderr)
         return 2
-    out = bench_lbvh(args)
+    if args.workload == "c2":
+        out = bench_c2(args)
+        if not args.no_extras:
+            out["c4"] = bench_c4(args)
+            out["lbvh"] = bench_lbvh(args)
+    elif args.workload == "c4":
+        r = bench_c4(args)
+        out = {"metric": METRIC, "value": r["batched_paths_per_s"], "unit": "paths/s",
+               "ms_per_step": r["batched_graph_ms"], "dtype": "f32+f64",
+               "config": {"workload": r["workload"]}, "c4": r}
+    else:
+        r = bench_lbvh(args)
+        out = {"metric": METRIC, "value": r["value"], "unit": "Mtri/s", "ms_per_step": r["ms_per_step"],
+               "dtype": "u32", "config": {"workload": r["workload"], "n_triangles": r["n_triangles"]},
+               "roofline": {"bound": "hbm", "achieved": r["achieved_GBps"], "peak": HBM_PEAK_GBPS,
+                            "unit": "GB/s", "frac": r["hbm_frac"], "traffic": None},
+               "cpu_baseline": r.get("cpu_baseline"), "lbvh": r}
     out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "data": "synthetic"})
     print(json.dumps(out), flush=True)

@@ -58,6 +58,26 @@ assert PATH_RESULT.itemsize == 48
 PATH_NOT_CONVERGED, PATH_CONVERGED, PATH_CONTRIBUTES = 0, 1, 2   # palz_path_result.status
 REFINE_NONE, REFINE_VERTICES, REFINE_ALL = 0, 1, 2                # palz_batched_solve refine modes
 
+# Arithmetic of one Newton iteration of the reference solver (solveConstraints,
+# pk.h:477-590: 2K single-tangent residual/Jacobian passes with the cost, then
+# the Householder QR step, the update and the triangle test), counted exactly
+# with a FLOP-counting scalar (+ - * / sqrt = 1).  A residual evaluation costs
+# EVAL_BASE[K] + EVAL_PER_T[K] * (number of refractive vertices); the step after
+# a non-converged evaluation costs at most STEP[K].  These are the algorithmic
+# FLOPs of the solver roofline (the kernels here share the primal chain across
+# tangents and execute fewer).
+ITERATION_FLOPS_EVAL_BASE = {1: 591, 2: 2322, 3: 5193}
+ITERATION_FLOPS_EVAL_PER_T = {1: 56, 2: 112, 3: 168}
+ITERATION_FLOPS_STEP = {1: 85, 2: 361, 3: 941}
+
+
+def iteration_flops(event: int) -> tuple[int, int]:
+    """(evaluation, step) FLOPs of one reference Newton iteration for an event."""
+    K = event >> 24
+    n_t = bin(event & ((1 << K) - 1)).count("1")
+    return ITERATION_FLOPS_EVAL_BASE[K] + ITERATION_FLOPS_EVAL_PER_T[K] * n_t, ITERATION_FLOPS_STEP[K]
+
+
 LIGHT_INTENSITY = 16.0   # m_lightIntencity (pk.h:1686)
 RADIANCE_CLAMP = 12.0    # m_radianceClamp (pk.h:1688)
 MIN_THROUGHPUT = 0.05    # m_minThroughput (pk.h:1687)
@@ -193,6 +213,11 @@ def download_paths(d_paths: DeviceBuffer, n: int) -> np.ndarray:
     return d_paths.download(PATH, n)
 
 
+def download_paths_at(ptr: int, start: int, n: int) -> np.ndarray:
+    """Paths [start, start + n) of a device path list."""
+    return download_ptr(ptr + start * PATH.itemsize, PATH, n)
+
+
 def download_results(d_results: DeviceBuffer, n: int) -> np.ndarray:
     return d_results.download(PATH_RESULT, n)
 
@@ -270,6 +295,16 @@ class BatchedSolver:
         call(name, C.c_void_p(self.handle), C.c_void_p(d_scenes.ptr), C.c_uint32(n_scenes), C.c_uint32(n_lambdas),
              int(refine), C.c_void_p(d_results.ptr if d_results else None),
              C.c_void_p(d_refined.ptr if d_refined else None), _sp(self.stream))
+
+    TIMED_KERNELS = ("newton_k1", "newton_k2", "newton_k3", "compact", "finish_k1", "finish_k2", "finish_k3")
+
+    def set_timing(self, enable: bool) -> None:
+        call("palz_batched_set_timing", C.c_void_p(self.handle), int(bool(enable)))
+
+    def timings(self) -> dict:
+        ms = (C.c_float * 7)()
+        call("palz_batched_timings", C.c_void_p(self.handle), ms)
+        return dict(zip(self.TIMED_KERNELS, (float(v) for v in ms)))
 
     def stats(self) -> dict:
         s = (C.c_uint64 * 16)()

@@ -13,10 +13,11 @@
 //     queue, so a wave never idles behind its slowest lane;
 //   * the Jacobian of an iteration comes from one multi-tangent dual pass
 //     (bit-identical to the reference's 2K single-tangent passes);
-//   * converged chains queue up for a finish kernel: throughput with K+1
-//     occlusion rays decides on the FP32 solution (the same decisions as the
-//     plain kernel), then one FP64 Newton step refines the vertices that feed
-//     dA/dw and the reported solution;
+//   * converged chains are compacted in item order (so the finish kernel's
+//     waves keep the tuple-coherent order of the path list for their
+//     occlusion rays) and finished: throughput with K+1 occlusion rays decides
+//     on the FP32 solution (the same decisions as the plain kernel), then one
+//     FP64 Newton step refines the converged vertices;
 //   * launch shapes are chosen per chain length by an autotuner, and a whole
 //     solve can be captured in a hipGraph (counters are reset by a kernel, no
 //     memset nodes).
@@ -24,6 +25,7 @@
 #include <cstring>
 
 #include "optics.hpp"
+#include "prims.hpp"
 #include "sort.hpp"
 
 namespace palz {
@@ -34,19 +36,19 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kMaxIterations = 32;          // main_gpu.cu:456
 constexpr float kCostTolerance = 1.0e-10f;  // main_gpu.cu:456
 constexpr int kWarmUp = 1;                  // main_gpu.cu:456
-constexpr int kQueueStride = 6;             // floats per queued solution (K <= 3)
+constexpr int kQueueStride = 6;             // floats per converged solution (K <= 3)
 
 struct Counters {
     uint32_t ticket[3];
-    uint32_t queue[3];
+    uint32_t n_items;    // items of every chain length (scan length)
+    uint32_t converged;  // scan total
+    uint32_t pad_;
     unsigned long long stats[16];
 };
 
-__global__ void k_reset(Counters* c) {
-    if (threadIdx.x < 3) {
-        c->ticket[threadIdx.x] = 0u;
-        c->queue[threadIdx.x] = 0u;
-    }
+__global__ void k_reset(Counters* c, const uint32_t* __restrict__ ranges, uint32_t L) {
+    if (threadIdx.x < 3) c->ticket[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) c->n_items = ranges[4] * L;
     if (threadIdx.x < 16) c->stats[threadIdx.x] = 0ull;
 }
 
@@ -124,7 +126,7 @@ template <int K>
 __global__ void __launch_bounds__(256)
     k_newton(const palz_scene_desc* __restrict__ scenes, uint32_t n_scenes, const palz_path* __restrict__ paths,
              const uint32_t* __restrict__ ranges, uint32_t L, uint32_t chunk, Counters* __restrict__ ctr,
-             uint32_t* __restrict__ q_item, float* __restrict__ q_params, int* __restrict__ q_iters,
+             uint32_t* __restrict__ flags, float* __restrict__ conv_params, int* __restrict__ conv_iters,
              palz_path_result* __restrict__ results) {
     constexpr int P = 2 * K;
     constexpr int R = 3 * K;
@@ -134,7 +136,6 @@ __global__ void __launch_bounds__(256)
     const int lane = threadIdx.x & 63;
     const uint64_t lt = __lanemask_lt();
     uint32_t* ticket = &ctr->ticket[K - 1];
-    uint32_t* q_count = &ctr->queue[K - 1];
 
     uint32_t item = kNone;
     int iter = 0;
@@ -187,6 +188,7 @@ __global__ void __launch_bounds__(256)
                     } else {
                         // a path naming a scene outside the batch is not solved
                         write_unconverged(results, item, 0, prm, P);
+                        flags[item] = 0u;
                         item = kNone;
                         atomicAdd(&ctr->stats[14], 1ull);
                     }
@@ -226,29 +228,23 @@ __global__ void __launch_bounds__(256)
                 }
                 if (outside) {
                     write_unconverged(results, item, iter, prm, P);
+                    flags[item] = 0u;
                     item = kNone;
                 } else if (++iter == kMaxIterations) {
                     write_unconverged(results, item, kMaxIterations, prm, P);
+                    flags[item] = 0u;
                     item = kNone;
                 }
             }
         }
-        // converged chains join the finish queue (one atomic per wave)
-        const uint64_t cm = __ballot(converged);
-        if (cm != 0ull) {
-            const int leader = __ffsll((unsigned long long)cm) - 1;
-            uint32_t base = 0u;
-            if (lane == leader) base = atomicAdd(q_count, (uint32_t)__popcll(cm));
-            base = (uint32_t)__shfl((int)base, leader, 64);
-            if (converged) {
-                const uint32_t slot = base + (uint32_t)__popcll(cm & lt);
-                q_item[slot] = item;
-                q_iters[slot] = iter;
+        // converged chains are stored per item for the compaction
+        if (converged) {
+            flags[item] = 1u;
+            conv_iters[item] = iter;
 #pragma unroll
-                for (int i = 0; i < P; ++i) q_params[(size_t)slot * kQueueStride + i] = prm[i];
-                ++n_conv;
-                item = kNone;
-            }
+            for (int i = 0; i < P; ++i) conv_params[(size_t)item * kQueueStride + i] = prm[i];
+            ++n_conv;
+            item = kNone;
         }
     }
     wave_add_stat(&ctr->stats[(K - 1) * 4 + 0], n_items);
@@ -257,18 +253,49 @@ __global__ void __launch_bounds__(256)
 }
 
 // Throughput, FP64 refinement, dA/dw and irradiance of the converged chains.
+__device__ __forceinline__ uint32_t compact_pos(const uint32_t* excl, uint32_t j, uint32_t n, uint32_t total) {
+    return j < n ? excl[j] : total;
+}
+
+// One Newton step in FP64 from the FP32 solution (two tangents per pass).
 template <int K>
-__global__ void __launch_bounds__(256)
-    k_finish(const palz_scene_desc* __restrict__ scenes, const palz_path* __restrict__ paths, uint32_t L,
-             Counters* __restrict__ ctr, const uint32_t* __restrict__ q_item, const float* __restrict__ q_params,
-             const int* __restrict__ q_iters, int refine, palz_path_result* __restrict__ results,
-             double* __restrict__ refined) {
+__device__ __forceinline__ void refine_step(const float* prm, v3 pb, v3 pe, const ChainTri* tri, float eta,
+                                            uint32_t event, double* p64) {
     constexpr int P = 2 * K;
     constexpr int R = 3 * K;
-    const uint32_t n = ctr->queue[K - 1];
+#pragma unroll
+    for (int i = 0; i < P; ++i) p64[i] = (double)prm[i];
+    double A[R * P];
+    double b[R];
+    double cost = 0.0;
+#pragma unroll
+    for (int first = 0; first < P; first += 2)
+        chain_residual<K, 2, double, d3>(p64, first, mkd(pb), mkd(pe), tri, (double)eta, event, A, b, &cost);
+    double dp[P];
+    solve_qr<R, P, double>(A, b, dp);
+#pragma unroll
+    for (int i = 0; i < P; ++i) p64[i] = p64[i] - dp[i];
+}
+
+// Throughput, dA/dw and irradiance of the converged chains of length K, in
+// item order.  kRefined: the FP64 step runs here and its chain feeds dA/dw
+// (PALZ_REFINE_ALL); otherwise dA/dw uses the FP32 chain.
+template <int K, bool kRefined>
+__global__ void __launch_bounds__(256)
+    k_finish(const palz_scene_desc* __restrict__ scenes, const palz_path* __restrict__ paths,
+             const uint32_t* __restrict__ ranges, uint32_t L, Counters* __restrict__ ctr,
+             const uint32_t* __restrict__ excl, const uint32_t* __restrict__ list,
+             const float* __restrict__ conv_params, const int* __restrict__ conv_iters,
+             palz_path_result* __restrict__ results, double* __restrict__ refined) {
+    constexpr int P = 2 * K;
+    const uint32_t n_items = ctr->n_items, total = ctr->converged;
+    const uint32_t begin = compact_pos(excl, ranges[K - 1] * L, n_items, total);
+    const uint32_t end = compact_pos(excl, ranges[K] * L, n_items, total);
+    __shared__ PacketStack packet_stacks[256 / 64];
+    PacketStack* pstack = &packet_stacks[threadIdx.x >> 6];
     unsigned long long n_contrib = 0ull;
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-        const uint32_t item = q_item[s];
+    for (uint32_t s = begin + blockIdx.x * blockDim.x + threadIdx.x; s < end; s += gridDim.x * blockDim.x) {
+        const uint32_t item = list[s];
         const uint32_t pi = item / L, l = item % L;
         const palz_path path = paths[pi];
         const palz_scene_desc* sc = scenes + path.scene;
@@ -277,35 +304,42 @@ __global__ void __launch_bounds__(256)
         for (int k = 0; k < K; ++k) tri[k] = load_chain_tri(sc->tris, sc->attribs, path.tris[k]);
         float prm[P];
 #pragma unroll
-        for (int i = 0; i < P; ++i) prm[i] = q_params[(size_t)s * kQueueStride + i];
+        for (int i = 0; i < P; ++i) prm[i] = conv_params[(size_t)item * kQueueStride + i];
         const palz_first_diffuse q = sc->points[path.pixel];
         const v3 pe = mk(q.p);
         const v3 pb = mk(sc->p_light);
         const float eta = item_eta(sc, l);
-        const uint32_t event = path.event;
-        const float thr =
-            contributable_throughput<K>(prm, pb, pe, tri, event, sc->nodes, sc->tris, *sc->root, eta);
+        const uint32_t event = path.event;  // per lane (see k_solve_plain)
+        // contributableThroughput: every test but the occlusion rays per lane,
+        // then the K + 1 occlusion rays as wave packets (per lane when the
+        // wave's items belong to more than one scene)
+        Segment seg[K + 1] = {};
+        float thr = throughput_unoccluded<K>(prm, pb, pe, tri, event, eta, seg);
+        const uint32_t scene0 = __builtin_amdgcn_readfirstlane(path.scene);
+        const bool one_scene = __ballot(path.scene != scene0) == 0ull;
+        bool visible = 0.0f < thr;
+#pragma unroll
+        for (int i = 0; i < K + 1; ++i) {
+            const v3 ird = inv_rd(seg[i].rd);
+            bool hit;
+            if (one_scene) {
+                const palz_scene_desc* us = scenes + scene0;
+                hit = any_hit_packet(us->nodes, us->tris, *us->root, visible, seg[i].ro, seg[i].rd, ird, 1.0f,
+                                     pstack);
+            } else {
+                hit = visible && any_hit_escape(sc->nodes, sc->tris, *sc->root, seg[i].ro, seg[i].rd, ird, 1.0f);
+            }
+            if (hit) visible = false;
+        }
+        if (!visible) thr = 0.0f;
         float pf[P];
 #pragma unroll
         for (int i = 0; i < P; ++i) pf[i] = prm[i];
-        if (refine != PALZ_REFINE_NONE) {
-            // one Newton step in FP64 from the FP32 solution (two tangents per pass)
+        if (kRefined) {
             double p64[P];
+            refine_step<K>(prm, pb, pe, tri, eta, event, p64);
 #pragma unroll
-            for (int i = 0; i < P; ++i) p64[i] = (double)prm[i];
-            double A[R * P];
-            double b[R];
-            double cost = 0.0;
-#pragma unroll
-            for (int first = 0; first < P; first += 2)
-                chain_residual<K, 2, double, d3>(p64, first, mkd(pb), mkd(pe), tri, (double)eta, event, A, b, &cost);
-            double dp[P];
-            solve_qr<R, P, double>(A, b, dp);
-#pragma unroll
-            for (int i = 0; i < P; ++i) {
-                p64[i] = p64[i] - dp[i];
-                pf[i] = (float)p64[i];
-            }
+            for (int i = 0; i < P; ++i) pf[i] = (float)p64[i];
             if (refined) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) refined[(size_t)item * 8 + i] = i < P ? p64[i] : 0.0;
@@ -313,17 +347,14 @@ __global__ void __launch_bounds__(256)
         }
         palz_path_result r;
         r.status = PALZ_PATH_CONVERGED;
-        r.iterations = q_iters[s];
+        r.iterations = conv_iters[item];
         r.contribution = 0.0f;
         r.footprint = 0.0f;
 #pragma unroll
         for (int i = 0; i < 8; ++i) r.params[i] = i < P ? pf[i] : 0.0f;
         if (0.0f < thr) {
-            // PALZ_REFINE_VERTICES keeps the irradiance on the FP32 chain (the
-            // reference's arithmetic); PALZ_REFINE_ALL feeds the refined one.
-            const float* pa = refine == PALZ_REFINE_ALL ? pf : prm;
-            const float area = dadw<K>(pb, chain_vertex(tri[0], pa[0], pa[1]) - pb, pe, tri, event, eta);
-            const v3 last = chain_vertex(tri[K - 1], pa[P - 2], pa[P - 1]);
+            const float area = dadw<K>(pb, chain_vertex(tri[0], pf[0], pf[1]) - pb, pe, tri, event, eta);
+            const v3 last = chain_vertex(tri[K - 1], pf[P - 2], pf[P - 1]);
             float c = thr * sc->light_intensity / area * hmax(dot(normalize(last - pe), mk(q.ng)), 0.0f);
             c = hmin(c, sc->radiance_clamp);
             atomicAdd(sc->image + (size_t)l * sc->n_points + path.pixel, c);
@@ -337,30 +368,95 @@ __global__ void __launch_bounds__(256)
     wave_add_stat(&ctr->stats[(K - 1) * 4 + 2], n_contrib);
 }
 
-// Items of chain length 4 are not part of a batch: mark and count them.
+// PALZ_REFINE_VERTICES: the FP64 step of every converged chain of length K,
+// after the finish kernel (its irradiance stays on the FP32 chain); writes the
+// refined barycentrics into the results and the FP64 output.
+template <int K>
+__global__ void __launch_bounds__(256)
+    k_refine(const palz_scene_desc* __restrict__ scenes, const palz_path* __restrict__ paths,
+             const uint32_t* __restrict__ ranges, uint32_t L, const Counters* __restrict__ ctr,
+             const uint32_t* __restrict__ excl, const uint32_t* __restrict__ list,
+             const float* __restrict__ conv_params, palz_path_result* __restrict__ results,
+             double* __restrict__ refined) {
+    constexpr int P = 2 * K;
+    const uint32_t n_items = ctr->n_items, total = ctr->converged;
+    const uint32_t begin = compact_pos(excl, ranges[K - 1] * L, n_items, total);
+    const uint32_t end = compact_pos(excl, ranges[K] * L, n_items, total);
+    for (uint32_t s = begin + blockIdx.x * blockDim.x + threadIdx.x; s < end; s += gridDim.x * blockDim.x) {
+        const uint32_t item = list[s];
+        const uint32_t pi = item / L, l = item % L;
+        const palz_path path = paths[pi];
+        const palz_scene_desc* sc = scenes + path.scene;
+        ChainTri tri[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) tri[k] = load_chain_tri(sc->tris, sc->attribs, path.tris[k]);
+        float prm[P];
+#pragma unroll
+        for (int i = 0; i < P; ++i) prm[i] = conv_params[(size_t)item * kQueueStride + i];
+        double p64[P];
+        refine_step<K>(prm, mk(sc->p_light), mk(sc->points[path.pixel].p), tri, item_eta(sc, l), path.event, p64);
+        if (results) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) results[item].params[i] = i < P ? (float)p64[i] : 0.0f;
+        }
+        if (refined) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) refined[(size_t)item * 8 + i] = i < P ? p64[i] : 0.0;
+        }
+    }
+}
+
+// Items of chain length 4 (and of none) are not part of a batch: mark and
+// count them; their flags stay clear for the compaction.
 __global__ void __launch_bounds__(256) k_skipped(const uint32_t* __restrict__ ranges, uint32_t L,
-                                                 Counters* __restrict__ ctr, palz_path_result* __restrict__ results) {
-    const uint32_t b = ranges[3] * L, e = ranges[4] * L;
+                                                 Counters* __restrict__ ctr, uint32_t* __restrict__ flags,
+                                                 palz_path_result* __restrict__ results) {
+    const uint32_t b = ranges[3] * L, e = ranges[4] * L, z = ranges[0] * L;
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr->stats[15] = (unsigned long long)(e - b);
-    if (!results) return;
-    for (uint32_t i = b + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {
-        palz_path_result r = {};
-        results[i] = r;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < z + (e - b); i += stride) {
+        const uint32_t item = i < z ? i : b + (i - z);
+        flags[item] = 0u;
+        if (results) {
+            palz_path_result r = {};
+            results[item] = r;
+        }
+    }
+}
+
+// Converged items, listed in item order from the exclusive scan of their flags.
+__global__ void __launch_bounds__(256) k_compact_scatter(const uint32_t* __restrict__ excl,
+                                                         const Counters* __restrict__ ctr,
+                                                         uint32_t* __restrict__ list) {
+    const uint32_t n = ctr->n_items, total = ctr->converged;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = excl[i];
+        const uint32_t next = i + 1 < n ? excl[i + 1] : total;
+        if (next != p) list[p] = i;
     }
 }
 
 template <int K>
 void launch_newton(const palz_launch_config& c, int cus, hipStream_t st, const palz_scene_desc* scenes,
                    uint32_t n_scenes, const palz_path* paths, const uint32_t* ranges, uint32_t L, Counters* ctr,
-                   uint32_t* qi, float* qp, int* qit, palz_path_result* results) {
+                   uint32_t* flags, float* cp, int* cit, palz_path_result* results) {
     k_newton<K><<<c.blocks_per_cu * cus, c.block, 0, st>>>(scenes, n_scenes, paths, ranges, L, (uint32_t)c.chunk,
-                                                          ctr, qi, qp, qit, results);
+                                                          ctr, flags, cp, cit, results);
 }
 template <int K>
 void launch_finish(const palz_launch_config& c, int cus, hipStream_t st, const palz_scene_desc* scenes,
-                   const palz_path* paths, uint32_t L, Counters* ctr, const uint32_t* qi, const float* qp,
-                   const int* qit, int refine, palz_path_result* results, double* refined) {
-    k_finish<K><<<cus * 8, c.finish_block, 0, st>>>(scenes, paths, L, ctr, qi, qp, qit, refine, results, refined);
+                   const palz_path* paths, const uint32_t* ranges, uint32_t L, Counters* ctr, const uint32_t* excl,
+                   const uint32_t* list, const float* cp, const int* cit, int refine, palz_path_result* results,
+                   double* refined) {
+    const dim3 grid(cus * 8), block(c.finish_block);
+    if (refine == PALZ_REFINE_ALL) {
+        k_finish<K, true><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, cit, results,
+                                                  refined);
+        return;
+    }
+    k_finish<K, false><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, cit, results, nullptr);
+    if (refine == PALZ_REFINE_VERTICES)
+        k_refine<K><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, results, refined);
 }
 
 }  // namespace
@@ -375,9 +471,13 @@ struct palz_batched {
     uint32_t* ranges = nullptr;  // [5] + [1] path count
     uint32_t cap_paths = 0;
     palz::Counters* ctr = nullptr;
-    uint32_t* q_item = nullptr;
-    float* q_params = nullptr;
-    int* q_iters = nullptr;
+    // per item: converged flag (scanned in place to list positions), FP32
+    // solution and iteration count; the compacted list of converged items
+    uint32_t* flags = nullptr;
+    float* conv_params = nullptr;
+    int* conv_iters = nullptr;
+    uint32_t* list = nullptr;
+    uint32_t* scan_sums = nullptr;
     size_t cap_items = 0;
     palz_launch_config cfg[3] = {{256, 4, 256, 128}, {256, 2, 256, 128}, {128, 2, 256, 128}};
     int cus = 256;
@@ -385,6 +485,9 @@ struct palz_batched {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t capture = nullptr;  // graphs are captured here (never the null stream)
+    bool timing = false;
+    bool timed_solves = false;
+    hipEvent_t ev[8] = {};
     struct Key {
         const void *scenes, *results, *refined;
         uint32_t n_scenes, L;
@@ -405,11 +508,14 @@ struct palz_batched {
     }
     int ensure_items(size_t n) {
         if (n <= cap_items) return PALZ_OK;
-        for (void* p : {(void*)q_item, (void*)q_params, (void*)q_iters})
+        drop_graph();  // a captured graph holds the old buffers
+        for (void* p : {(void*)flags, (void*)conv_params, (void*)conv_iters, (void*)list, (void*)scan_sums})
             if (p) PALZ_HIP(hipFree(p));
-        PALZ_HIP(hipMalloc(&q_item, sizeof(uint32_t) * n));
-        PALZ_HIP(hipMalloc(&q_params, sizeof(float) * palz::kQueueStride * n));
-        PALZ_HIP(hipMalloc(&q_iters, sizeof(int) * n));
+        PALZ_HIP(hipMalloc(&flags, sizeof(uint32_t) * n));
+        PALZ_HIP(hipMalloc(&conv_params, sizeof(float) * palz::kQueueStride * n));
+        PALZ_HIP(hipMalloc(&conv_iters, sizeof(int) * n));
+        PALZ_HIP(hipMalloc(&list, sizeof(uint32_t) * n));
+        PALZ_HIP(hipMalloc(&scan_sums, sizeof(uint32_t) * palz::scan_scratch_entries((uint32_t)n)));
         cap_items = n;
         return PALZ_OK;
     }
@@ -422,24 +528,55 @@ struct palz_batched {
     ~palz_batched() {
         drop_graph();
         if (capture) (void)hipStreamDestroy(capture);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
         for (void* p : {(void*)paths, (void*)order, (void*)words, (void*)words_tmp, (void*)ranges, (void*)ctr,
-                        (void*)q_item, (void*)q_params, (void*)q_iters})
+                        (void*)flags, (void*)conv_params, (void*)conv_iters, (void*)list, (void*)scan_sums})
             if (p) (void)hipFree(p);
     }
 
-    int launch_all(const palz_scene_desc* scenes, uint32_t n_scenes, uint32_t L, int refine,
-                   palz_path_result* results, double* refined, hipStream_t st) {
+    template <int K>
+    void newton(const palz_launch_config& c, const palz_scene_desc* scenes, uint32_t n_scenes, uint32_t L,
+                palz_path_result* results, hipStream_t st) {
+        palz::launch_newton<K>(c, cus, st, scenes, n_scenes, paths, ranges, L, ctr, flags, conv_params, conv_iters,
+                               results);
+    }
+    template <int K>
+    void finish(const palz_launch_config& c, const palz_scene_desc* scenes, uint32_t L, int refine,
+                palz_path_result* results, double* refined, hipStream_t st) {
+        palz::launch_finish<K>(c, cus, st, scenes, paths, ranges, L, ctr, flags, list, conv_params, conv_iters,
+                               refine, results, refined);
+    }
+    // K = 4 marks, the scan of the converged flags and the scatter into the list
+    void compact(uint32_t L, palz_path_result* results, hipStream_t st) {
         using namespace palz;
-        k_reset<<<1, 64, 0, st>>>(ctr);
-        const palz_path* P = paths;
-        launch_newton<1>(cfg[0], cus, st, scenes, n_scenes, P, ranges, L, ctr, q_item, q_params, q_iters, results);
-        launch_finish<1>(cfg[0], cus, st, scenes, P, L, ctr, q_item, q_params, q_iters, refine, results, refined);
-        launch_newton<2>(cfg[1], cus, st, scenes, n_scenes, P, ranges, L, ctr, q_item, q_params, q_iters, results);
-        launch_finish<2>(cfg[1], cus, st, scenes, P, L, ctr, q_item, q_params, q_iters, refine, results, refined);
-        launch_newton<3>(cfg[2], cus, st, scenes, n_scenes, P, ranges, L, ctr, q_item, q_params, q_iters, results);
-        launch_finish<3>(cfg[2], cus, st, scenes, P, L, ctr, q_item, q_params, q_iters, refine, results, refined);
-        k_skipped<<<64, 256, 0, st>>>(ranges, L, ctr, results);
+        k_skipped<<<256, 256, 0, st>>>(ranges, L, ctr, flags, results);
+        launch_exclusive_scan(flags, flags, (uint32_t)cap_items, &ctr->n_items, &ctr->converged, scan_sums, st);
+        k_compact_scatter<<<cus * 8, 256, 0, st>>>(flags, ctr, list);
+    }
+
+    int launch_all(const palz_scene_desc* scenes, uint32_t n_scenes, uint32_t L, int refine,
+                   palz_path_result* results, double* refined, hipStream_t st, bool timed) {
+        using namespace palz;
+        const bool t = timed && timing;
+        k_reset<<<1, 64, 0, st>>>(ctr, ranges, L);
+        if (t) PALZ_HIP(hipEventRecord(ev[0], st));
+        newton<1>(cfg[0], scenes, n_scenes, L, results, st);
+        if (t) PALZ_HIP(hipEventRecord(ev[1], st));
+        newton<2>(cfg[1], scenes, n_scenes, L, results, st);
+        if (t) PALZ_HIP(hipEventRecord(ev[2], st));
+        newton<3>(cfg[2], scenes, n_scenes, L, results, st);
+        if (t) PALZ_HIP(hipEventRecord(ev[3], st));
+        compact(L, results, st);
+        if (t) PALZ_HIP(hipEventRecord(ev[4], st));
+        finish<1>(cfg[0], scenes, L, refine, results, refined, st);
+        if (t) PALZ_HIP(hipEventRecord(ev[5], st));
+        finish<2>(cfg[1], scenes, L, refine, results, refined, st);
+        if (t) PALZ_HIP(hipEventRecord(ev[6], st));
+        finish<3>(cfg[2], scenes, L, refine, results, refined, st);
+        if (t) PALZ_HIP(hipEventRecord(ev[7], st));
         PALZ_LAUNCH_CHECK();
+        if (t) timed_solves = true;
         return PALZ_OK;
     }
 };
@@ -549,7 +686,7 @@ int palz_batched_solve(palz_batched* b, const palz_scene_desc* d_scenes, uint32_
                        int refine, palz_path_result* d_results, double* d_refined, void* stream) {
     int rc = check_solve_args(b, d_scenes, n_scenes, n_lambdas, refine);
     if (rc != PALZ_OK) return rc;
-    return b->launch_all(d_scenes, n_scenes, n_lambdas, refine, d_results, d_refined, (hipStream_t)stream);
+    return b->launch_all(d_scenes, n_scenes, n_lambdas, refine, d_results, d_refined, (hipStream_t)stream, true);
 }
 
 int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, uint32_t n_scenes,
@@ -564,7 +701,7 @@ int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, u
     if (!same) {
         b->drop_graph();
         PALZ_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
-        rc = b->launch_all(d_scenes, n_scenes, n_lambdas, refine, d_results, d_refined, b->capture);
+        rc = b->launch_all(d_scenes, n_scenes, n_lambdas, refine, d_results, d_refined, b->capture, false);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(b->capture, &g);
         if (rc != PALZ_OK) return rc;
@@ -574,6 +711,24 @@ int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, u
         b->key = k;
     }
     PALZ_HIP(hipGraphLaunch(b->exec, st));
+    return PALZ_OK;
+}
+
+int palz_batched_set_timing(palz_batched* b, int enable) {
+    if (!b) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_set_timing: NULL");
+    if (enable && !b->ev[0])
+        for (hipEvent_t& e : b->ev) PALZ_HIP(hipEventCreate(&e));
+    b->timing = enable != 0;
+    b->timed_solves = false;
+    return PALZ_OK;
+}
+
+int palz_batched_timings(palz_batched* b, float* ms7) {
+    if (!b || !ms7) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_timings: NULL");
+    if (!b->timing || !b->timed_solves)
+        return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_timings: no timed solve since timing was enabled");
+    PALZ_HIP(hipEventSynchronize(b->ev[7]));
+    for (int i = 0; i < 7; ++i) PALZ_HIP(hipEventElapsedTime(&ms7[i], b->ev[i], b->ev[i + 1]));
     return PALZ_OK;
 }
 
@@ -601,67 +756,79 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
     const int blocks[3] = {64, 128, 256};
     const int per_cu[4] = {1, 2, 4, 8};
     const int chunks[3] = {64, 256, 1024};
+    auto elapsed = [&]() -> float {
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return ms;
+    };
+    auto run_newton = [&](int K, const palz_launch_config& c) {
+        if (K == 1) b->newton<1>(c, d_scenes, n_scenes, n_lambdas, nullptr, st);
+        if (K == 2) b->newton<2>(c, d_scenes, n_scenes, n_lambdas, nullptr, st);
+        if (K == 3) b->newton<3>(c, d_scenes, n_scenes, n_lambdas, nullptr, st);
+    };
+    float newton_ms[3] = {0.0f, 0.0f, 0.0f}, finish_ms[3] = {0.0f, 0.0f, 0.0f};
+    // Newton kernels: block size, resident blocks per CU and queue chunk
     for (int K = 1; K <= 3; ++K) {
+        if (ranges[K] == ranges[K - 1]) continue;
         palz_launch_config best = b->cfg[K - 1];
-        float best_ms = 0.0f, best_fin = 0.0f;
-        if (ranges[K] > ranges[K - 1]) {
-            best_ms = 1e30f;
-            auto time_newton = [&](const palz_launch_config& c) -> float {
-                k_reset<<<1, 64, 0, st>>>(b->ctr);
-                (void)hipEventRecord(e0, st);
-                if (K == 1) launch_newton<1>(c, b->cus, st, d_scenes, n_scenes, b->paths, b->ranges, n_lambdas, b->ctr, b->q_item, b->q_params, b->q_iters, nullptr);
-                if (K == 2) launch_newton<2>(c, b->cus, st, d_scenes, n_scenes, b->paths, b->ranges, n_lambdas, b->ctr, b->q_item, b->q_params, b->q_iters, nullptr);
-                if (K == 3) launch_newton<3>(c, b->cus, st, d_scenes, n_scenes, b->paths, b->ranges, n_lambdas, b->ctr, b->q_item, b->q_params, b->q_iters, nullptr);
-                (void)hipEventRecord(e1, st);
-                (void)hipEventSynchronize(e1);
-                float ms = 0.0f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                return ms;
-            };
-            for (int blk : blocks)
-                for (int pc : per_cu) {
-                    if (blk * pc > 2048) continue;  // at most 32 waves resident per CU
-                    for (int ch : chunks) {
-                        palz_launch_config c = best;
-                        c.block = blk;
-                        c.blocks_per_cu = pc;
-                        c.chunk = ch;
-                        const float ms = time_newton(c);
-                        if (ms < best_ms) {
-                            best_ms = ms;
-                            best = c;
-                        }
+        float best_ms = 1e30f;
+        for (int blk : blocks)
+            for (int pc : per_cu) {
+                // at least one wave per SIMD, at most 32 waves resident per CU
+                if (blk * pc < 256 || blk * pc > 2048) continue;
+                for (int ch : chunks) {
+                    palz_launch_config c = best;
+                    c.block = blk;
+                    c.blocks_per_cu = pc;
+                    c.chunk = ch;
+                    k_reset<<<1, 64, 0, st>>>(b->ctr, b->ranges, n_lambdas);
+                    (void)hipEventRecord(e0, st);
+                    run_newton(K, c);
+                    const float ms = elapsed();
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = c;
                     }
                 }
-            // finish kernel block size, on the queue of the best Newton shape
-            (void)time_newton(best);
-            best_fin = 1e30f;
-            for (int fb : blocks) {
-                palz_launch_config c = best;
-                c.finish_block = fb;
-                (void)hipEventRecord(e0, st);
-                if (K == 1) launch_finish<1>(c, b->cus, st, d_scenes, b->paths, n_lambdas, b->ctr, b->q_item, b->q_params, b->q_iters, refine, nullptr, nullptr);
-                if (K == 2) launch_finish<2>(c, b->cus, st, d_scenes, b->paths, n_lambdas, b->ctr, b->q_item, b->q_params, b->q_iters, refine, nullptr, nullptr);
-                if (K == 3) launch_finish<3>(c, b->cus, st, d_scenes, b->paths, n_lambdas, b->ctr, b->q_item, b->q_params, b->q_iters, refine, nullptr, nullptr);
-                (void)hipEventRecord(e1, st);
-                (void)hipEventSynchronize(e1);
-                float ms = 0.0f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (ms < best_fin) {
-                    best_fin = ms;
-                    best.finish_block = fb;
-                }
             }
-            b->cfg[K - 1] = best;
+        b->cfg[K - 1] = best;
+        newton_ms[K - 1] = best_ms;
+    }
+    // finish kernels: block size, on the converged list of the tuned Newton shapes
+    k_reset<<<1, 64, 0, st>>>(b->ctr, b->ranges, n_lambdas);
+    for (int K = 1; K <= 3; ++K) run_newton(K, b->cfg[K - 1]);
+    b->compact(n_lambdas, nullptr, st);
+    for (int K = 1; K <= 3; ++K) {
+        if (ranges[K] == ranges[K - 1]) continue;
+        float best_fin = 1e30f;
+        palz_launch_config& best = b->cfg[K - 1];
+        for (int fb : blocks) {
+            palz_launch_config c = best;
+            c.finish_block = fb;
+            (void)hipEventRecord(e0, st);
+            if (K == 1) b->finish<1>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
+            if (K == 2) b->finish<2>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
+            if (K == 3) b->finish<3>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
+            const float ms = elapsed();
+            if (ms < best_fin) {
+                best_fin = ms;
+                best.finish_block = fb;
+            }
         }
-        if (report) {
+        finish_ms[K - 1] = best_fin;
+    }
+    if (report) {
+        for (int K = 1; K <= 3; ++K) {
+            const palz_launch_config& c = b->cfg[K - 1];
             int32_t* r = report + (K - 1) * 6;
-            r[0] = best.block;
-            r[1] = best.blocks_per_cu;
-            r[2] = best.chunk;
-            r[3] = best.finish_block;
-            r[4] = (int32_t)(best_ms * 1000.0f);
-            r[5] = (int32_t)(best_fin * 1000.0f);
+            r[0] = c.block;
+            r[1] = c.blocks_per_cu;
+            r[2] = c.chunk;
+            r[3] = c.finish_block;
+            r[4] = (int32_t)(newton_ms[K - 1] * 1000.0f);
+            r[5] = (int32_t)(finish_ms[K - 1] * 1000.0f);
         }
     }
     (void)hipEventDestroy(e0);

@@ -625,6 +625,163 @@ __host__ __device__ inline bool occluded(const palz_internal_node* nodes, const 
 // getVertex (pk.h:752-758)
 PALZ_HD v3 chain_vertex(const ChainTri& c, float u, float v) { return c.v0 + (c.v1 - c.v0) * u + (c.v2 - c.v0) * v; }
 
+// The occlusion ray of occluded() (pk.h:88-115): origin and direction of the
+// offset segment, tested over t in [0, 1].
+struct Segment {
+    v3 ro, rd;
+};
+PALZ_HD Segment occlusion_segment(v3 from, v3 from_n, v3 to, v3 to_n) {
+    if (dot(to - from, from_n) < 0.0f) from_n = -from_n;
+    if (dot(from - to, to_n) < 0.0f) to_n = -to_n;
+    const v3 from_safe = from + from_n * ray_offset_scale(from);
+    const v3 to_safe = to + to_n * ray_offset_scale(to);
+    return {from_safe, to_safe - from_safe};
+}
+
+// Any-hit traversal of one occlusion ray per lane, run by the whole wave as a
+// packet: the wave walks the tree once, loading each node with scalar loads,
+// and every lane tests its own ray against the node's child boxes; a subtree
+// is entered with the mask of the lanes whose ray hit its box (and every box
+// above it), so each lane tests exactly the leaves the escape-link traversal
+// (any_hit_escape) would test for it, in another order.  Any-hit is
+// independent of the order, so the result equals any_hit_escape's.  `stack`
+// is this wave's LDS stack; if it would overflow, the lanes still pending fall
+// back to any_hit_escape.  Called by every lane of the wave that reaches it
+// (lanes with query == false only take part in the ballots).
+constexpr int kPacketDepth = 64;
+struct PacketStack {
+    uint64_t mask[kPacketDepth];
+    uint32_t node[kPacketDepth];
+};
+
+__device__ inline bool any_hit_packet(const palz_internal_node* __restrict__ nodes,
+                                      const palz_triangle* __restrict__ tris, uint32_t root, bool query, v3 ro,
+                                      v3 rd, v3 ird, float t_max, PacketStack* stack) {
+    const uint64_t me = 1ull << (threadIdx.x & 63);
+    bool hit = false;
+    uint64_t pending = __ballot(query);
+    if (pending == 0ull) return false;
+    auto test_leaf = [&](uint32_t leaf, uint64_t lanes) {
+        if (lanes & me) {
+            const palz_triangle tr = ld_tri(tris, leaf & ~PALZ_NODE_LEAF_BIT);
+            float t, u, v;
+            v3 ng;
+            if (intersect_tri(&t, &u, &v, &ng, 0.0f, t_max, ro, rd, mk(tr.vs[0]), mk(tr.vs[1]), mk(tr.vs[2])))
+                hit = true;
+        }
+        pending &= ~__ballot(hit);
+    };
+    if (root & PALZ_NODE_LEAF_BIT) {
+        test_leaf(root, pending);
+        return hit;
+    }
+    int sp = 0;
+    bool overflow = false;
+    uint32_t node = root;
+    uint64_t mask = pending;
+    for (;;) {
+        const uint32_t un = __builtin_amdgcn_readfirstlane(node);
+        const palz_internal_node* nd = nodes + un;
+        const uint32_t c0 = nd->children[0], c1 = nd->children[1];
+        const palz_aabb b0 = nd->aabbs[0], b1 = nd->aabbs[1];
+        bool h0 = false, h1 = false;
+        if (mask & me) {
+            float a, b;
+            slabs(ro, ird, mk(b0.lower), mk(b0.upper), t_max, &a, &b);
+            h0 = a <= b;
+            slabs(ro, ird, mk(b1.lower), mk(b1.upper), t_max, &a, &b);
+            h1 = a <= b;
+        }
+        uint64_t m0 = __ballot(h0), m1 = __ballot(h1);
+        if (m0 != 0ull && (c0 & PALZ_NODE_LEAF_BIT)) {
+            test_leaf(c0, m0);
+            m0 = 0ull;
+        }
+        if (m1 != 0ull && (c1 & PALZ_NODE_LEAF_BIT)) {
+            test_leaf(c1, m1);
+            m1 = 0ull;
+        }
+        m0 &= pending;
+        m1 &= pending;
+        if (m0 != 0ull && m1 != 0ull) {
+            if (sp == kPacketDepth) {
+                overflow = true;
+                break;
+            }
+            if ((threadIdx.x & 63) == __builtin_amdgcn_readfirstlane(threadIdx.x & 63)) {
+                stack->mask[sp] = m1;
+                stack->node[sp] = c1;
+            }
+            ++sp;
+            node = c0;
+            mask = m0;
+        } else if (m0 != 0ull) {
+            node = c0;
+            mask = m0;
+        } else if (m1 != 0ull) {
+            node = c1;
+            mask = m1;
+        } else {
+            mask = 0ull;
+            while (sp > 0 && mask == 0ull) {
+                --sp;
+                __builtin_amdgcn_wave_barrier();
+                node = stack->node[sp];
+                mask = stack->mask[sp] & pending;
+            }
+            if (mask == 0ull) break;
+        }
+        if (pending == 0ull) break;
+    }
+    if (overflow && (pending & me)) hit = any_hit_escape(nodes, tris, root, ro, rd, ird, t_max);
+    return hit;
+}
+
+// contributableThroughput<K> (pk.h:760-842) without its occlusion rays: the
+// throughput of a chain that passes every other test (0 otherwise) and, when
+// it is positive, the K + 1 occlusion segments to test.
+template <int K>
+__host__ __device__ inline float throughput_unoccluded(const float* params, v3 p_beg, v3 p_end, const ChainTri* tri,
+                                                      uint32_t event, float eta, Segment* seg) {
+    v3 vert[K + 2];
+    v3 sn[K + 2];
+    vert[0] = p_beg;
+    vert[K + 1] = p_end;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const float u = params[2 * k], v = params[2 * k + 1];
+        if (u < 0.0f || v < 0.0f || 1.0f < u + v) return 0.0f;
+        vert[k + 1] = chain_vertex(tri[k], u, v);
+        sn[k + 1] = normalize(tri[k].s0 + (tri[k].s1 - tri[k].s0) * u + (tri[k].s2 - tri[k].s0) * v);
+    }
+    sn[0] = normalize(vert[1] - vert[0]);
+    sn[K + 1] = normalize(vert[K] - vert[K + 1]);
+    float throughput = 1.0f;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const v3 wi = vert[k] - vert[k + 1];
+        const v3 wo = vert[k + 2] - vert[k + 1];
+        const v3 n = sn[k + 1];
+        const bool e_T = !(0.0f < dot(wi, n) * dot(wo, n));
+        if (ev_is_T(event, k) != e_T) return 0.0f;
+        const v3 ng = cross(tri[k].v1 - tri[k].v0, tri[k].v2 - tri[k].v0);  // unnormalizedNormalOf
+        const bool eg_T = !(0.0f < dot(wi, ng) * dot(wo, ng));
+        if (e_T != eg_T) return 0.0f;
+        if (tri[k].material == PALZ_MATERIAL_DIELECTRIC) {
+            const float r = fresnel_exact_norm_free(wi, n, eta);
+            throughput *= e_T ? 1.0f - r : r;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < K + 1; ++i) {
+        const v3 d = vert[i] - vert[i + 1];
+        if (dot(d, d) < 1.0e-5f) return 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < K + 1; ++i) seg[i] = occlusion_segment(vert[i], sn[i], vert[i + 1], sn[i + 1]);
+    return throughput;
+}
+
 // contributableThroughput<K> (pk.h:760-842)
 template <int K>
 __host__ __device__ inline float contributable_throughput(const float* params, v3 p_beg, v3 p_end, const ChainTri* tri,
