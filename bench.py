@@ -48,6 +48,9 @@ STAGE_BYTES_PER_TRI = {
     "sort_pass2": 16.0, "sort_pass3": 16.0, "deltas": 9.0, "build": 41.0 + 64.0,
 }
 LBVH_BYTES_PER_TRI = 266.0  # 202 B/tri + 64 B per internal node
+LBVH_KERNEL = {"scene_aabb": "k_scene_aabb", "morton_hist": "k_morton_hist", "sort_pass0": "k_sort_pass<false>",
+               "sort_pass1": "k_sort_pass<false>", "sort_pass2": "k_sort_pass<false>",
+               "sort_pass3": "k_sort_pass<false>", "deltas": "k_deltas", "build": "k_build"}
 
 
 def log(msg: str) -> None:
@@ -76,6 +79,37 @@ def measure_copy_bandwidth(nbytes: int = 1 << 30, reps: int = 5) -> float:
 
 # ---------------------------------------------------------------------------------------------------
 # spectral slab (headline)
+
+def NEWTON_BYTES(K: int) -> int:
+    """Compulsory HBM bytes per (path, wavelength) item of k_newton<K>: the
+    path record (64), its K triangles (36 each) and attributes (40 each), the
+    receiver point (40) and wavelength (4); written: the converged flag (4),
+    solution (8K) and iteration count (4), or a 48-byte result record
+    (the larger of the two, 48, is counted)."""
+    return 64 + 76 * K + 40 + 4 + 48
+
+
+def FINISH_BYTES(K: int) -> int:
+    """Compulsory HBM bytes per converged item of k_finish<K>: the list entry
+    (4), path record (64), FP32 solution (8K) and iteration count (4), K
+    triangles and attributes (76 each), receiver point (40), wavelength (4),
+    the 48-byte result record and the image word (4).  BVH nodes and triangles
+    read by the occlusion rays are not counted (the BVH stays in cache)."""
+    return 4 + 64 + 8 * K + 4 + 76 * K + 40 + 4 + 48 + 4
+
+
+def profiled_traffic(workload: str, kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == workload and cells[1] == f"`{kernel}`":
+            return float(cells[4]) * 1e6
+    return None
+
 
 def _solver_flops(stats: dict, event_share: dict) -> dict:
     """Algorithmic FLOPs of the Newton kernels per chain length: residual
@@ -233,25 +267,36 @@ def bench_c2(args) -> dict:
     # the plain kernel's chains are the FP32 ones (bit-identical to the oracle's)
     plain_value = contributing * sample["vertex"]["fp32_frac_within_1e-5"] / (ms_plain * 1e-3)
 
-    # roofline of the dominant kernel (the solver is VALU-bound, FP32, no MFMA)
+    # roofline of the dominant kernel.  The path uses no MFMA; the roofline is
+    # the HBM one over each kernel's compulsory bytes per item (FINISH_BYTES /
+    # NEWTON_BYTES), next to the Newton kernels' FP32 rate over the reference
+    # iteration's FLOPs.
     event_share = {}
     for K, ev, d, n in plain_lists:
         event_share[ev] = event_share.get(ev, 0) + n
     flops = _solver_flops(stats, event_share)
     dominant = max(kern, key=kern.get)
-    achieved = None
-    if dominant.startswith("newton"):
-        K = int(dominant[-1])
-        achieved = flops[K] / (kern[dominant] * 1e-3) / 1e12
-    label = {"compact": "k_skipped + exclusive scan + k_compact_scatter"}.get(
-        dominant, f"k_newton<{dominant[-1]}>" if dominant.startswith("newton") else
-        f"k_finish<{dominant[-1]}> + k_refine<{dominant[-1]}>")
-    roof = {"bound": "valu", "kernel": label,
-            "achieved": round(achieved, 2) if achieved is not None else None, "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4) if achieved is not None else None,
-            "traffic": None,
-            "flops_per_launch": flops.get(int(dominant[-1])) if dominant.startswith("newton") else None,
-            "kernel_ms": round(kern[dominant], 3)}
+    units = {}
+    for K in (1, 2, 3):
+        row = stats[f"K{K}"]
+        units[f"newton_k{K}"] = (row["items"], NEWTON_BYTES(K))
+        units[f"finish_k{K}"] = (row["converged"], FINISH_BYTES(K))
+    kernel_name = {f"newton_k{K}": f"k_newton<{K}>" for K in (1, 2, 3)}
+    kernel_name.update({f"finish_k{K}": f"k_finish<{K}, false>" for K in (1, 2, 3)})
+    kernel_name.update({"compact": "k_compact_scatter", "refine": "k_refine<K>"})
+    roof = {"bound": "hbm", "kernel": kernel_name[dominant], "kernel_ms": round(kern[dominant], 3)}
+    if dominant in units:
+        n_units, per_unit = units[dominant]
+        achieved = n_units * per_unit / (kern[dominant] * 1e-3) / 1e9
+        roof.update({"achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "bytes_per_unit": per_unit, "units": n_units})
+    traffic = profiled_traffic("c2", kernel_name[dominant])
+    roof["traffic"] = traffic
+    valu = {f"k_newton<{K}>": {"achieved_TFLOPs": round(flops[K] / (kern[f"newton_k{K}"] * 1e-3) / 1e12, 2),
+                               "peak_TFLOPs": FP32_PEAK_TFLOPS,
+                               "frac": round(flops[K] / (kern[f"newton_k{K}"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+                               "flops": flops[K]}
+            for K in (1, 2, 3) if kern[f"newton_k{K}"] > 0 and flops[K] > 0}
     setup_s = time.perf_counter() - t_setup
     return {
         "metric": METRIC,
@@ -273,8 +318,7 @@ def bench_c2(args) -> dict:
                     "kernel_ms": {k: round(v, 3) for k, v in kern.items()},
                     "converged_items": converged, "contributing_items": contributing,
                     "stats": stats, "launch_configs": tune, "wall_s": round(wall, 3),
-                    "tflops_by_K": {K: round(flops[K] / (kern[f"newton_k{K}"] * 1e-3) / 1e12, 2)
-                                    for K in (1, 2, 3) if kern[f"newton_k{K}"] > 0}},
+                    "newton_fp32_roofline": valu},
         "plain": {"value": round(plain_value, 1), "unit": "paths/s", "ms_per_step": round(ms_plain, 3),
                   "launches": len(plain_lists), "kernel_ms": {k: round(v, 3) for k, v in plain_kernel_ms.items()},
                   "batched_over_plain": round(value / plain_value, 2) if plain_value else None,
@@ -512,10 +556,14 @@ def main() -> int:
                "config": {"workload": r["workload"]}, "c4": r}
     else:
         r = bench_lbvh(args)
+        dom = max(r["stage_ms"], key=r["stage_ms"].get)
+        ach = r["stage_GBps"][dom]
         out = {"metric": METRIC, "value": r["value"], "unit": "Mtri/s", "ms_per_step": r["ms_per_step"],
                "dtype": "u32", "config": {"workload": r["workload"], "n_triangles": r["n_triangles"]},
-               "roofline": {"bound": "hbm", "achieved": r["achieved_GBps"], "peak": HBM_PEAK_GBPS,
-                            "unit": "GB/s", "frac": r["hbm_frac"], "traffic": None},
+               "roofline": {"bound": "hbm", "kernel": LBVH_KERNEL[dom], "achieved": ach, "peak": HBM_PEAK_GBPS,
+                            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
+                            "bytes_per_unit": STAGE_BYTES_PER_TRI[dom], "units": r["n_triangles"],
+                            "traffic": profiled_traffic("lbvh 16M", LBVH_KERNEL[dom])},
                "cpu_baseline": r.get("cpu_baseline"), "lbvh": r}
     out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "data": "synthetic"})

@@ -349,10 +349,11 @@ int palz_batched_stats(palz_batched* b, uint64_t* stats16, void* stream);
 /* Kernel timing of eager solves (graph launches are not timed): when
  * enabled, each palz_batched_solve records HIP events on its stream around
  * its kernels; palz_batched_timings waits for the last solve's events and
- * returns 7 durations in ms: Newton K=1, Newton K=2, Newton K=3, the
- * compaction of the converged items, finish K=1, finish K=2, finish K=3. */
+ * returns 8 durations in ms: Newton K=1, Newton K=2, Newton K=3, the
+ * compaction of the converged items, finish K=1, finish K=2, finish K=3, and
+ * the FP64 refinement of all three (PALZ_REFINE_VERTICES; 0 otherwise). */
 int palz_batched_set_timing(palz_batched* b, int enable);
-int palz_batched_timings(palz_batched* b, float* ms7);
+int palz_batched_timings(palz_batched* b, float* ms8);
 /* Time every candidate launch shape per chain length on the prepared batch
  * and keep the fastest (the autotuner).  The image receives the candidates'
  * contributions; clear it afterwards.  Writes the chosen shapes and their

@@ -455,8 +455,12 @@ void launch_finish(const palz_launch_config& c, int cus, hipStream_t st, const p
         return;
     }
     k_finish<K, false><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, cit, results, nullptr);
-    if (refine == PALZ_REFINE_VERTICES)
-        k_refine<K><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, results, refined);
+}
+template <int K>
+void launch_refine(const palz_launch_config& c, int cus, hipStream_t st, const palz_scene_desc* scenes,
+                   const palz_path* paths, const uint32_t* ranges, uint32_t L, Counters* ctr, const uint32_t* excl,
+                   const uint32_t* list, const float* cp, palz_path_result* results, double* refined) {
+    k_refine<K><<<cus * 8, c.finish_block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, results, refined);
 }
 
 }  // namespace
@@ -487,7 +491,7 @@ struct palz_batched {
     hipStream_t capture = nullptr;  // graphs are captured here (never the null stream)
     bool timing = false;
     bool timed_solves = false;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[9] = {};
     struct Key {
         const void *scenes, *results, *refined;
         uint32_t n_scenes, L;
@@ -547,6 +551,13 @@ struct palz_batched {
         palz::launch_finish<K>(c, cus, st, scenes, paths, ranges, L, ctr, flags, list, conv_params, conv_iters,
                                refine, results, refined);
     }
+    template <int K>
+    void refine_vertices(const palz_launch_config& c, const palz_scene_desc* scenes, uint32_t L, int refine,
+                         palz_path_result* results, double* refined, hipStream_t st) {
+        if (refine == PALZ_REFINE_VERTICES)
+            palz::launch_refine<K>(c, cus, st, scenes, paths, ranges, L, ctr, flags, list, conv_params, results,
+                                   refined);
+    }
     // K = 4 marks, the scan of the converged flags and the scatter into the list
     void compact(uint32_t L, palz_path_result* results, hipStream_t st) {
         using namespace palz;
@@ -575,6 +586,10 @@ struct palz_batched {
         if (t) PALZ_HIP(hipEventRecord(ev[6], st));
         finish<3>(cfg[2], scenes, L, refine, results, refined, st);
         if (t) PALZ_HIP(hipEventRecord(ev[7], st));
+        refine_vertices<1>(cfg[0], scenes, L, refine, results, refined, st);
+        refine_vertices<2>(cfg[1], scenes, L, refine, results, refined, st);
+        refine_vertices<3>(cfg[2], scenes, L, refine, results, refined, st);
+        if (t) PALZ_HIP(hipEventRecord(ev[8], st));
         PALZ_LAUNCH_CHECK();
         if (t) timed_solves = true;
         return PALZ_OK;
@@ -723,12 +738,12 @@ int palz_batched_set_timing(palz_batched* b, int enable) {
     return PALZ_OK;
 }
 
-int palz_batched_timings(palz_batched* b, float* ms7) {
-    if (!b || !ms7) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_timings: NULL");
+int palz_batched_timings(palz_batched* b, float* ms8) {
+    if (!b || !ms8) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_timings: NULL");
     if (!b->timing || !b->timed_solves)
         return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_timings: no timed solve since timing was enabled");
-    PALZ_HIP(hipEventSynchronize(b->ev[7]));
-    for (int i = 0; i < 7; ++i) PALZ_HIP(hipEventElapsedTime(&ms7[i], b->ev[i], b->ev[i + 1]));
+    PALZ_HIP(hipEventSynchronize(b->ev[8]));
+    for (int i = 0; i < 8; ++i) PALZ_HIP(hipEventElapsedTime(&ms8[i], b->ev[i], b->ev[i + 1]));
     return PALZ_OK;
 }
 
@@ -811,6 +826,9 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
             if (K == 1) b->finish<1>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
             if (K == 2) b->finish<2>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
             if (K == 3) b->finish<3>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
+            if (K == 1) b->refine_vertices<1>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
+            if (K == 2) b->refine_vertices<2>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
+            if (K == 3) b->refine_vertices<3>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
             const float ms = elapsed();
             if (ms < best_fin) {
                 best_fin = ms;

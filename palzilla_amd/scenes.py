@@ -280,3 +280,23 @@ def triangle_soup(n: int, seed: int = 3, mu: float = -6.0, sigma: float = 0.5) -
                      [-0.5 * r, -0.5, 0.0]])
     v = np.einsum("nij,kj->nki", rot, base) * s[:, None, None] + c[:, None, :]
     return v.reshape(n, 9).astype(np.float32)
+
+
+def load_obj(path: str) -> np.ndarray:
+    """Triangles of a Wavefront OBJ file, (n, 9) float32: `v` records, and
+    each `f` record fan-triangulated in file order (v0, vi, vi+1)."""
+    verts, tris = [], []
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if not parts:
+                continue
+            if parts[0] == "v":
+                verts.append([float(x) for x in parts[1:4]])
+            elif parts[0] == "f":
+                idx = [int(p.split("/")[0]) for p in parts[1:]]
+                idx = [i - 1 if i > 0 else len(verts) + i for i in idx]
+                for j in range(1, len(idx) - 1):
+                    tris.append(idx[0:1] + idx[j:j + 2])
+    v = np.asarray(verts, dtype=np.float32)
+    return v[np.asarray(tris, dtype=np.int64)].reshape(-1, 9)

@@ -1,5 +1,7 @@
 """GPU parity of the onesweep sort and the LBVH build (bit-exact vs the CPU
 oracle and numpy's stable argsort)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -132,3 +134,16 @@ def test_lbvh_heightfield_grid(sorter):
         b = _gpu_lbvh(sorter, tris, bits)
         assert b.download_root() == root
         assert b.download_nodes().tobytes() == nodes.tobytes()
+
+
+@pytest.mark.parametrize("bits", [30, 32])
+def test_lbvh_reference_mesh_bit_exact(sorter, bits):
+    """The reference's own mesh (bin/assets/table/table.obj), bit for bit."""
+    tris = scenes.load_obj(os.path.join(os.path.dirname(__file__), "golden", "table.obj"))
+    nodes, root, words, deltas, box = oracle.lbvh(tris, bits=bits)
+    b = _gpu_lbvh(sorter, tris, bits)
+    assert b.download_scene_aabb().tobytes() == box.tobytes()
+    g_words, g_deltas = b.download_sorted()
+    assert np.array_equal(g_words, words) and np.array_equal(g_deltas, deltas)
+    assert b.download_root() == root
+    assert b.download_nodes().tobytes() == nodes.tobytes()
