@@ -213,6 +213,19 @@ def bench_c2(args) -> dict:
     ms = t0.elapsed_ms(t1) / args.steps
     stats = bs.stats()
 
+    # the same solve with the per-tuple interval culling in front of Newton
+    bs.set_culling(True)
+    bs.set_timing(True)
+    bs.solve(d_scenes, 1, L, CA.REFINE_VERTICES, d_res)
+    cull_kern = bs.timings()
+    cull_stats = bs.stats()
+    bs.set_timing(False)
+    bs.set_culling(False)
+    culling = {"culled_items": cull_stats["culled"], "items": n_items,
+               "cull_kernel_ms": round(cull_kern["cull"], 3),
+               "step_ms_with_culling": round(sum(cull_kern.values()), 3),
+               "contributing_items_with_culling": sum(cull_stats[f"K{k}"]["contributing"] for k in (1, 2, 3))}
+
     # the same solve as one hipGraph launch
     g.clear_image()
     bs.solve(d_scenes, 1, L, CA.REFINE_VERTICES, d_res, graph=True)
@@ -283,7 +296,7 @@ def bench_c2(args) -> dict:
         units[f"finish_k{K}"] = (row["converged"], FINISH_BYTES(K))
     kernel_name = {f"newton_k{K}": f"k_newton<{K}>" for K in (1, 2, 3)}
     kernel_name.update({f"finish_k{K}": f"k_finish<{K}, false>" for K in (1, 2, 3)})
-    kernel_name.update({"compact": "k_compact_scatter", "refine": "k_refine<K>"})
+    kernel_name.update({"compact": "k_compact_scatter", "refine": "k_refine<K>", "cull": "k_cull_paths"})
     roof = {"bound": "hbm", "kernel": kernel_name[dominant], "kernel_ms": round(kern[dominant], 3)}
     if dominant in units:
         n_units, per_unit = units[dominant]
@@ -324,6 +337,7 @@ def bench_c2(args) -> dict:
                   "batched_over_plain": round(value / plain_value, 2) if plain_value else None,
                   "batched_over_plain_time": round(ms_plain / ms, 2),
                   "image_rel_rmse_batched_vs_plain": img_rmse},
+        "interval_culling": culling,
         "vertex_error": sample["vertex"],
         "parity_sample": sample["parity"],
         "pipeline_ms": {k: round(v, 3) for k, v in pipeline_ms.items()},

@@ -267,7 +267,7 @@ typedef struct {
 } palz_scene_desc;
 
 /* Per (path, wavelength) outcome, item index = path * n_lambdas + l. */
-enum { PALZ_PATH_NOT_CONVERGED = 0, PALZ_PATH_CONVERGED = 1, PALZ_PATH_CONTRIBUTES = 2 };
+enum { PALZ_PATH_NOT_CONVERGED = 0, PALZ_PATH_CONVERGED = 1, PALZ_PATH_CONTRIBUTES = 2, PALZ_PATH_CULLED = 3 };
 typedef struct {
     uint32_t status;
     int32_t iterations;   /* Newton iterations run (index of the exit iteration) */
@@ -285,6 +285,18 @@ typedef struct {
 int palz_solve_specular(int K, uint32_t event, const palz_scene_desc* scene, const palz_path* d_paths,
                         uint32_t n_paths, const uint32_t* d_count, palz_path_result* d_results,
                         void* stream);
+
+/* Per-tuple path-cuts culling: the admissibility chain of
+ * traverseAdmissibleNodes at leaf level (pk.h:2101-2207, interval.h:34-713) for
+ * every path of chain length 1-3, with the triangles' AABBs as vertex boxes,
+ * the AABBs of their shading normals as normal boxes and eta spanning the
+ * scene's wavelengths, in interval arithmetic with directed rounding.
+ * d_admissible[p] = 0 only if no point of the boxes brings every vertex's
+ * Newton residual within the solver's tolerance, so a culled path has no
+ * contributing solution; else 1 (also for other chain lengths and for paths
+ * naming a scene >= n_scenes).  `d_count` (optional, device) caps n_paths. */
+int palz_interval_cull(const palz_scene_desc* d_scenes, uint32_t n_scenes, const palz_path* d_paths,
+                       uint32_t n_paths, const uint32_t* d_count, uint8_t* d_admissible, void* stream);
 
 /* ---- batched persistent solver (no reference counterpart) ---------------
  * Replaces the reference's per-event-descriptor launch loop (pk.h:1480-1586,
@@ -342,18 +354,25 @@ int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, u
                              void* stream);
 /* Per chain length K = 1..3 (row K-1): [0] items, [1] converged, [2]
  * contributing, [3] residual evaluations (Newton iterations plus the final
- * converged check); row 3: [2] items whose path names a scene >= n_scenes
- * (not solved, status NOT_CONVERGED), [3] items skipped (K = 4).
+ * converged check); row 3: [0] items culled, [2] items whose path names a
+ * scene >= n_scenes (not solved, status NOT_CONVERGED), [3] items skipped
+ * (K = 4).
  * Synchronises the stream. */
 int palz_batched_stats(palz_batched* b, uint64_t* stats16, void* stream);
 /* Kernel timing of eager solves (graph launches are not timed): when
  * enabled, each palz_batched_solve records HIP events on its stream around
  * its kernels; palz_batched_timings waits for the last solve's events and
- * returns 8 durations in ms: Newton K=1, Newton K=2, Newton K=3, the
- * compaction of the converged items, finish K=1, finish K=2, finish K=3, and
- * the FP64 refinement of all three (PALZ_REFINE_VERTICES; 0 otherwise). */
+ * returns 9 durations in ms: Newton K=1, Newton K=2, Newton K=3, the
+ * compaction of the converged items, finish K=1, finish K=2, finish K=3, the
+ * FP64 refinement of all three (PALZ_REFINE_VERTICES; else ~0) and the
+ * interval culling (when enabled; else ~0). */
 int palz_batched_set_timing(palz_batched* b, int enable);
-int palz_batched_timings(palz_batched* b, float* ms8);
+int palz_batched_timings(palz_batched* b, float* ms9);
+/* Per-tuple interval culling (palz_interval_cull) in front of the Newton
+ * kernels: items of a culled path get status PALZ_PATH_CULLED without a
+ * solve.  A culled path has no contributing solution, so the image and the
+ * contributing items are those of an unculled solve.  Off by default. */
+int palz_batched_set_culling(palz_batched* b, int enable);
 /* Time every candidate launch shape per chain length on the prepared batch
  * and keep the fastest (the autotuner).  The image receives the candidates'
  * contributions; clear it afterwards.  Writes the chosen shapes and their

@@ -33,6 +33,9 @@ _SIGS = {
     "orc_solve_paths": (None, [_P, _P, C.c_uint32, _P, C.c_int]),
     "orc_solve_f64": (C.c_int, [_P, _P, _P, _P, _P, C.c_double, C.c_double, C.c_int, _P, _P]),
     "orc_count_flops": (C.c_int, [_P, _P, _P, _P, _P, C.c_float, _P]),
+    "orc_iv_op": (None, [C.c_int, C.c_float, C.c_float, _P]),
+    "orc_iv_residual": (C.c_int, [_P, _P, _P, C.c_int, C.c_float, C.c_float, _P]),
+    "orc_interval_cull": (None, [_P, _P, _P, C.c_uint32, _P, _P, C.c_float, C.c_float, _P]),
     "orc_max_stack_depth": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
 }
 
@@ -229,3 +232,32 @@ def count_flops(scene, path, p_end, eta: float):
                                  _ptr(out)):
         raise ValueError("chain length must be 1..4")
     return int(out[0]), int(out[1])
+
+
+IV_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4}
+
+
+def iv_op(op: str, a: float, b: float = 0.0):
+    """(rounded down, rounded up) of a op b in float32 (directed rounding)."""
+    out = np.zeros(2, dtype=np.float32)
+    lib().orc_iv_op(IV_OPS[op], float(a), float(b), _ptr(out))
+    return float(out[0]), float(out[1])
+
+
+def iv_residual(wi_box, wo_box, n_box, refract: bool, eta_lo: float, eta_hi: float):
+    """Enclosure (3, 2) of cross(wo, wopt) over boxes given as (lo xyz, hi xyz),
+    or None where the refraction box may hold total internal reflection."""
+    args = [np.ascontiguousarray(np.asarray(b, dtype=np.float32).reshape(6)) for b in (wi_box, wo_box, n_box)]
+    out = np.zeros(6, dtype=np.float32)
+    ok = lib().orc_iv_residual(*(_ptr(a) for a in args), int(refract), float(eta_lo), float(eta_hi), _ptr(out))
+    return out.reshape(3, 2) if ok else None
+
+
+def interval_cull(scene, paths: np.ndarray, eta_lo: float, eta_hi: float) -> np.ndarray:
+    """Per-tuple admissibility (1/0) of paths of one scene."""
+    paths = np.ascontiguousarray(paths)
+    out = np.zeros(paths.shape[0], dtype=np.uint8)
+    pl = np.asarray(scene.p_light, dtype=np.float32)
+    lib().orc_interval_cull(_ptr(scene.tris), _ptr(scene.attribs), _ptr(paths), paths.shape[0], _ptr(pl),
+                            _ptr(scene.points), float(eta_lo), float(eta_hi), _ptr(out))
+    return out

@@ -14,7 +14,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["lbvh.cpp", "path.cpp"]
+SOURCES = ["lbvh.cpp", "path.cpp", "interval.cpp"]
 OUT = os.path.join(HERE, "liboracle.so")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-march=x86-64-v3", "-ffp-contract=off",
          "-fno-fast-math", "-Wall", "-Wextra", "-Wno-unused-parameter", "-pthread"]

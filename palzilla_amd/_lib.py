@@ -64,6 +64,8 @@ _SIGS = {
                                       C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "palz_solve_specular": (C.c_int, [C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                       C.c_void_p, C.c_void_p]),
+    "palz_interval_cull": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]),
     "palz_batched_create": (C.c_int, [c_void_pp]),
     "palz_batched_destroy": (C.c_int, [C.c_void_p]),
     "palz_batched_prepare": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -77,6 +79,7 @@ _SIGS = {
     "palz_batched_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p]),
     "palz_batched_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "palz_batched_timings": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "palz_batched_set_culling": (C.c_int, [C.c_void_p, C.c_int]),
     "palz_batched_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int,
                                         C.POINTER(C.c_int32), C.c_void_p]),
     "palz_lbvh_set_timing": (C.c_int, [C.c_void_p, C.c_int]),

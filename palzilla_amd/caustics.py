@@ -55,7 +55,7 @@ PATH_RESULT = np.dtype([("status", "<u4"), ("iterations", "<i4"), ("contribution
                         ("footprint", "<f4")])
 assert PATH_RESULT.itemsize == 48
 
-PATH_NOT_CONVERGED, PATH_CONVERGED, PATH_CONTRIBUTES = 0, 1, 2   # palz_path_result.status
+PATH_NOT_CONVERGED, PATH_CONVERGED, PATH_CONTRIBUTES, PATH_CULLED = 0, 1, 2, 3   # palz_path_result.status
 REFINE_NONE, REFINE_VERTICES, REFINE_ALL = 0, 1, 2                # palz_batched_solve refine modes
 
 # Arithmetic of one Newton iteration of the reference solver (solveConstraints,
@@ -297,15 +297,18 @@ class BatchedSolver:
              C.c_void_p(d_refined.ptr if d_refined else None), _sp(self.stream))
 
     TIMED_KERNELS = ("newton_k1", "newton_k2", "newton_k3", "compact", "finish_k1", "finish_k2", "finish_k3",
-                     "refine")
+                     "refine", "cull")
 
     def set_timing(self, enable: bool) -> None:
         call("palz_batched_set_timing", C.c_void_p(self.handle), int(bool(enable)))
 
     def timings(self) -> dict:
-        ms = (C.c_float * 8)()
+        ms = (C.c_float * 9)()
         call("palz_batched_timings", C.c_void_p(self.handle), ms)
         return dict(zip(self.TIMED_KERNELS, (float(v) for v in ms)))
+
+    def set_culling(self, enable: bool) -> None:
+        call("palz_batched_set_culling", C.c_void_p(self.handle), int(bool(enable)))
 
     def stats(self) -> dict:
         s = (C.c_uint64 * 16)()
@@ -313,6 +316,8 @@ class BatchedSolver:
         a = np.frombuffer(s, dtype=np.uint64).copy()
         out = {f"K{k + 1}": dict(zip(STAT_ROWS, (int(v) for v in a[4 * k:4 * k + 4]))) for k in range(3)}
         out["skipped"] = int(a[15])
+        out["culled"] = int(a[12])
+        out["bad_scene"] = int(a[14])
         return out
 
     def autotune(self, d_scenes: DeviceBuffer, n_scenes: int, n_lambdas: int, refine: int = 1) -> list[dict]:
@@ -371,3 +376,13 @@ class SceneBatch:
     def clear_images(self) -> None:
         for g in self.scenes:
             g.clear_image()
+
+
+def interval_cull(d_scenes: DeviceBuffer, n_scenes: int, d_paths: DeviceBuffer | int, n_paths: int,
+                  stream=None) -> np.ndarray:
+    """palz_interval_cull: per-path admissibility flags (1 = keep)."""
+    ptr = d_paths.ptr if isinstance(d_paths, DeviceBuffer) else d_paths
+    d_out = DeviceBuffer(max(n_paths, 1))
+    call("palz_interval_cull", C.c_void_p(d_scenes.ptr), C.c_uint32(n_scenes), C.c_void_p(ptr), C.c_uint32(n_paths),
+         C.c_void_p(None), C.c_void_p(d_out.ptr), _sp(stream))
+    return d_out.download(np.uint8, n_paths, stream)

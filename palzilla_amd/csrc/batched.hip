@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "cull.hpp"
 #include "optics.hpp"
 #include "prims.hpp"
 #include "sort.hpp"
@@ -103,10 +104,11 @@ __device__ __forceinline__ float item_eta(const palz_scene_desc* sc, uint32_t l)
 }
 
 __device__ __forceinline__ void write_unconverged(palz_path_result* results, uint32_t item, int iters,
-                                                  const float* prm, int P) {
+                                                  const float* prm, int P,
+                                                  uint32_t status = PALZ_PATH_NOT_CONVERGED) {
     if (!results) return;
     palz_path_result r;
-    r.status = PALZ_PATH_NOT_CONVERGED;
+    r.status = status;
     r.iterations = iters;
     r.contribution = 0.0f;
     r.footprint = 0.0f;
@@ -127,7 +129,7 @@ __global__ void __launch_bounds__(256)
     k_newton(const palz_scene_desc* __restrict__ scenes, uint32_t n_scenes, const palz_path* __restrict__ paths,
              const uint32_t* __restrict__ ranges, uint32_t L, uint32_t chunk, Counters* __restrict__ ctr,
              uint32_t* __restrict__ flags, float* __restrict__ conv_params, int* __restrict__ conv_iters,
-             palz_path_result* __restrict__ results) {
+             const uint8_t* __restrict__ admissible, palz_path_result* __restrict__ results) {
     constexpr int P = 2 * K;
     constexpr int R = 3 * K;
     const uint32_t first_path = ranges[K - 1];
@@ -146,7 +148,7 @@ __global__ void __launch_bounds__(256)
     ChainTri tri[K];
     uint32_t c_next = 0u, c_end = 0u;  // this wave's window of the work queue
     bool exhausted = total == 0u;
-    unsigned long long n_iter = 0ull, n_conv = 0ull, n_items = 0ull;
+    unsigned long long n_iter = 0ull, n_conv = 0ull, n_items = 0ull, n_culled = 0ull;
 
     for (;;) {
         // refill idle lanes, in lane order, from the wave's queue window
@@ -175,7 +177,13 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                     for (int i = 0; i < P; ++i) prm[i] = path.params[i];
                     ++n_items;
-                    if (path.scene < n_scenes) {
+                    if (admissible && !admissible[pi]) {
+                        // no point of the tuple's boxes can satisfy the chain
+                        write_unconverged(results, item, 0, prm, P, PALZ_PATH_CULLED);
+                        flags[item] = 0u;
+                        item = kNone;
+                        ++n_culled;
+                    } else if (path.scene < n_scenes) {
                         const palz_scene_desc* sc = scenes + path.scene;
 #pragma unroll
                         for (int k = 0; k < K; ++k) tri[k] = load_chain_tri(sc->tris, sc->attribs, path.tris[k]);
@@ -250,6 +258,7 @@ __global__ void __launch_bounds__(256)
     wave_add_stat(&ctr->stats[(K - 1) * 4 + 0], n_items);
     wave_add_stat(&ctr->stats[(K - 1) * 4 + 1], n_conv);
     wave_add_stat(&ctr->stats[(K - 1) * 4 + 3], n_iter);
+    wave_add_stat(&ctr->stats[12], n_culled);
 }
 
 // Throughput, FP64 refinement, dA/dw and irradiance of the converged chains.
@@ -439,9 +448,9 @@ __global__ void __launch_bounds__(256) k_compact_scatter(const uint32_t* __restr
 template <int K>
 void launch_newton(const palz_launch_config& c, int cus, hipStream_t st, const palz_scene_desc* scenes,
                    uint32_t n_scenes, const palz_path* paths, const uint32_t* ranges, uint32_t L, Counters* ctr,
-                   uint32_t* flags, float* cp, int* cit, palz_path_result* results) {
+                   uint32_t* flags, float* cp, int* cit, const uint8_t* admissible, palz_path_result* results) {
     k_newton<K><<<c.blocks_per_cu * cus, c.block, 0, st>>>(scenes, n_scenes, paths, ranges, L, (uint32_t)c.chunk,
-                                                          ctr, flags, cp, cit, results);
+                                                          ctr, flags, cp, cit, admissible, results);
 }
 template <int K>
 void launch_finish(const palz_launch_config& c, int cus, hipStream_t st, const palz_scene_desc* scenes,
@@ -491,18 +500,22 @@ struct palz_batched {
     hipStream_t capture = nullptr;  // graphs are captured here (never the null stream)
     bool timing = false;
     bool timed_solves = false;
-    hipEvent_t ev[9] = {};
+    hipEvent_t ev[10] = {};
+    bool cull = false;
+    uint8_t* admissible = nullptr;  // per sorted path, when culling
     struct Key {
         const void *scenes, *results, *refined;
         uint32_t n_scenes, L;
         int refine;
         palz_launch_config cfg[3];
+        int cull;
     } key{};
 
     int ensure_paths(uint32_t n) {
         if (n <= cap_paths) return PALZ_OK;
-        for (void* p : {(void*)paths, (void*)order, (void*)words, (void*)words_tmp})
+        for (void* p : {(void*)paths, (void*)order, (void*)words, (void*)words_tmp, (void*)admissible})
             if (p) PALZ_HIP(hipFree(p));
+        PALZ_HIP(hipMalloc(&admissible, n));
         PALZ_HIP(hipMalloc(&paths, sizeof(palz_path) * n));
         PALZ_HIP(hipMalloc(&order, sizeof(uint32_t) * n));
         PALZ_HIP(hipMalloc(&words, sizeof(uint64_t) * n));
@@ -534,7 +547,8 @@ struct palz_batched {
         if (capture) (void)hipStreamDestroy(capture);
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
-        for (void* p : {(void*)paths, (void*)order, (void*)words, (void*)words_tmp, (void*)ranges, (void*)ctr,
+        for (void* p : {(void*)paths, (void*)order, (void*)words, (void*)words_tmp, (void*)admissible,
+                        (void*)ranges, (void*)ctr,
                         (void*)flags, (void*)conv_params, (void*)conv_iters, (void*)list, (void*)scan_sums})
             if (p) (void)hipFree(p);
     }
@@ -543,7 +557,7 @@ struct palz_batched {
     void newton(const palz_launch_config& c, const palz_scene_desc* scenes, uint32_t n_scenes, uint32_t L,
                 palz_path_result* results, hipStream_t st) {
         palz::launch_newton<K>(c, cus, st, scenes, n_scenes, paths, ranges, L, ctr, flags, conv_params, conv_iters,
-                               results);
+                               cull ? admissible : nullptr, results);
     }
     template <int K>
     void finish(const palz_launch_config& c, const palz_scene_desc* scenes, uint32_t L, int refine,
@@ -572,6 +586,8 @@ struct palz_batched {
         const bool t = timed && timing;
         k_reset<<<1, 64, 0, st>>>(ctr, ranges, L);
         if (t) PALZ_HIP(hipEventRecord(ev[0], st));
+        if (cull) launch_cull_paths(scenes, n_scenes, paths, cap_paths, ranges + 5, admissible, st);
+        if (t) PALZ_HIP(hipEventRecord(ev[9], st));
         newton<1>(cfg[0], scenes, n_scenes, L, results, st);
         if (t) PALZ_HIP(hipEventRecord(ev[1], st));
         newton<2>(cfg[1], scenes, n_scenes, L, results, st);
@@ -710,7 +726,7 @@ int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, u
     int rc = check_solve_args(b, d_scenes, n_scenes, n_lambdas, refine);
     if (rc != PALZ_OK) return rc;
     hipStream_t st = (hipStream_t)stream;
-    palz_batched::Key k{d_scenes, d_results, d_refined, n_scenes, n_lambdas, refine, {}};
+    palz_batched::Key k{d_scenes, d_results, d_refined, n_scenes, n_lambdas, refine, {}, b->cull ? 1 : 0};
     std::copy(b->cfg, b->cfg + 3, k.cfg);
     const bool same = b->exec && std::memcmp(&k, &b->key, sizeof(k)) == 0;
     if (!same) {
@@ -738,12 +754,21 @@ int palz_batched_set_timing(palz_batched* b, int enable) {
     return PALZ_OK;
 }
 
-int palz_batched_timings(palz_batched* b, float* ms8) {
+int palz_batched_timings(palz_batched* b, float* ms8) {  // 9 entries
     if (!b || !ms8) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_timings: NULL");
     if (!b->timing || !b->timed_solves)
         return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_timings: no timed solve since timing was enabled");
     PALZ_HIP(hipEventSynchronize(b->ev[8]));
-    for (int i = 0; i < 8; ++i) PALZ_HIP(hipEventElapsedTime(&ms8[i], b->ev[i], b->ev[i + 1]));
+    PALZ_HIP(hipEventElapsedTime(&ms8[0], b->ev[9], b->ev[1]));
+    for (int i = 1; i < 8; ++i) PALZ_HIP(hipEventElapsedTime(&ms8[i], b->ev[i], b->ev[i + 1]));
+    PALZ_HIP(hipEventElapsedTime(&ms8[8], b->ev[0], b->ev[9]));
+    return PALZ_OK;
+}
+
+int palz_batched_set_culling(palz_batched* b, int enable) {
+    if (!b) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_set_culling: NULL");
+    b->cull = enable != 0;
+    b->drop_graph();
     return PALZ_OK;
 }
 
@@ -784,6 +809,7 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
         if (K == 3) b->newton<3>(c, d_scenes, n_scenes, n_lambdas, nullptr, st);
     };
     float newton_ms[3] = {0.0f, 0.0f, 0.0f}, finish_ms[3] = {0.0f, 0.0f, 0.0f};
+    if (b->cull) launch_cull_paths(d_scenes, n_scenes, b->paths, b->cap_paths, b->ranges + 5, b->admissible, st);
     // Newton kernels: block size, resident blocks per CU and queue chunk
     for (int K = 1; K <= 3; ++K) {
         if (ranges[K] == ranges[K - 1]) continue;

@@ -268,3 +268,42 @@ def test_multi_scene_batch():
             a, b = got[f].reshape(got.shape[0], -1), want[f].reshape(want.shape[0], -1)
             assert np.all((a == b) | (np.isnan(a) & np.isnan(b))), f"scene {i} field {f}"
         assert np.allclose(g.image().reshape(-1), hs.image, rtol=1e-5, atol=1e-6), f"scene {i} image"
+
+
+def _eta_range(s):
+    lam = s.lambdas.astype(np.float32)
+    eta = np.float32(s.cauchy_a) + np.float32(s.cauchy_b) / (lam * lam)
+    return float(eta.min()), float(eta.max())
+
+
+@pytest.mark.parametrize("far", [False, True])
+def test_interval_cull_matches_oracle(run, far):
+    paths = run.paths.copy()
+    if far:  # receiver points far from the tuples' caustics
+        n_pts = run.s.points.shape[0]
+        paths["pixel"] = (paths["pixel"] + n_pts // 2 + run.s.width // 2) % n_pts
+    d_paths = DeviceBuffer.from_array(paths)
+    got = CA.interval_cull(run.d_scenes, 1, d_paths, paths.shape[0])
+    want = oracle.interval_cull(run.s, paths, *_eta_range(run.s))
+    K = paths["event"] >> 24
+    want[K > 3] = 1
+    assert np.array_equal(got, want), f"{np.sum(got != want)} of {len(got)} flags differ"
+    print(f"{run.name} far={far}: culled {np.sum(got == 0)} of {len(got)}")
+
+
+def test_batched_culling_keeps_every_contribution(run):
+    run.bs.set_culling(True)
+    try:
+        res, _, img = run.solve(refine=CA.REFINE_NONE)
+        st = run.bs.stats()
+    finally:
+        run.bs.set_culling(False)
+    keep, want, want_img = run.oracle()
+    got = res[np.repeat(keep, run.L)]
+    culled = got["status"] == CA.PATH_CULLED
+    assert st["culled"] == int(culled.sum())
+    assert not np.any(want["status"][culled] == CA.PATH_CONTRIBUTES)
+    for f in FIELDS:
+        a, b = got[f][~culled], want[f][~culled]
+        assert a.tobytes() == b.tobytes(), f
+    assert np.allclose(img, want_img, rtol=1e-5, atol=1e-6)

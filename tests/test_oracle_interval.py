@@ -1,0 +1,183 @@
+"""The oracle's interval arithmetic and per-tuple culling, pinned without
+its own formulas: directed rounding against exact rational arithmetic,
+residual enclosures against FP64 evaluation at sampled points, and culling
+against the solver (a culled path never contributes)."""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from palzilla_amd import caustics as CA
+from palzilla_amd import scenes
+
+
+def _rd_ru(exact: Fraction):
+    """The float32 values just below/above (or equal to) an exact rational."""
+    f = np.float32(float(exact))  # round to nearest (via double; fixed below)
+    lo = hi = f
+    while Fraction(float(lo)) > exact:
+        lo = np.nextafter(lo, np.float32(-np.inf))
+    while Fraction(float(lo)) < exact and Fraction(float(np.nextafter(lo, np.float32(np.inf)))) <= exact:
+        lo = np.nextafter(lo, np.float32(np.inf))
+    while Fraction(float(hi)) < exact:
+        hi = np.nextafter(hi, np.float32(np.inf))
+    while Fraction(float(hi)) > exact and Fraction(float(np.nextafter(hi, np.float32(-np.inf)))) >= exact:
+        hi = np.nextafter(hi, np.float32(-np.inf))
+    return float(lo), float(hi)
+
+
+def _samples(rng, n):
+    mant = rng.uniform(1.0, 2.0, n)
+    expo = rng.integers(-40, 40, n)
+    sign = rng.choice([-1.0, 1.0], n)
+    return (sign * mant * 2.0 ** expo).astype(np.float32)
+
+
+@pytest.mark.parametrize("op", ["add", "sub", "mul", "div"])
+def test_directed_rounding_is_the_correct_rounding(op):
+    rng = np.random.default_rng({"add": 1, "sub": 2, "mul": 3, "div": 4}[op])
+    a, b = _samples(rng, 400), _samples(rng, 400)
+    a[:40] = b[:40] * np.float32(3.0)  # exact cases
+    for x, y in zip(a.tolist(), b.tolist()):
+        fx, fy = Fraction(x), Fraction(y)
+        exact = {"add": fx + fy, "sub": fx - fy, "mul": fx * fy, "div": fx / fy}[op]
+        lo, hi = oracle.iv_op(op, x, y)
+        want = _rd_ru(exact)
+        if abs(exact) < Fraction(2) ** -100:  # the tiny range is only bracketed
+            assert Fraction(lo) <= exact <= Fraction(hi)
+            continue
+        assert (lo, hi) == want, (op, x, y)
+
+
+def test_directed_sqrt_brackets_exactly():
+    rng = np.random.default_rng(5)
+    for x in np.abs(_samples(rng, 400)).tolist() + [4.0, 2.0, 0.25]:
+        lo, hi = oracle.iv_op("sqrt", x)
+        assert Fraction(lo) ** 2 <= Fraction(x) <= Fraction(hi) ** 2
+        # tight: one ulp apart at most, equal when the root is exact
+        assert np.nextafter(np.float32(lo), np.float32(np.inf)) >= np.float32(hi)
+    assert oracle.iv_op("sqrt", 4.0) == (2.0, 2.0)
+
+
+def _box(rng, scale, width):
+    c = rng.normal(0, scale, 3)
+    w = rng.uniform(0, width, 3)
+    return np.concatenate([c - w, c + w]).astype(np.float32)
+
+
+def _residual64(wi, wo, n, refract, eta):
+    if np.dot(wi, n) < 0:
+        pass  # the enclosure is of one orientation: the caller passes it
+    non = np.dot(n, n)
+    win = np.dot(wi, n)
+    if refract:
+        k = non * np.dot(wi, wi) * (eta * eta - 1.0) + win * win
+        if k < 0:
+            return None
+        wopt = -wi * non + n * (win - np.sqrt(k))
+    else:
+        wopt = n * (2.0 * win / non) - wi
+    return np.cross(wo, wopt)
+
+
+@pytest.mark.parametrize("refract", [False, True])
+def test_residual_enclosure_contains_sampled_points(refract):
+    rng = np.random.default_rng(11 + refract)
+    checked = 0
+    for _ in range(300):
+        wi_b = _box(rng, 1.0, 0.05)
+        wo_b = _box(rng, 1.0, 0.05)
+        n_b = _box(rng, 0.6, 0.02)
+        eta_lo, eta_hi = sorted(rng.uniform(1.3, 1.8, 2).astype(np.float32).tolist())
+        enc = oracle.iv_residual(wi_b, wo_b, n_b, refract, eta_lo, eta_hi)
+        if enc is None:
+            continue
+        for _ in range(20):
+            t = rng.random((3, 3))
+            wi = wi_b[:3] + (wi_b[3:] - wi_b[:3]) * t[0]
+            wo = wo_b[:3] + (wo_b[3:] - wo_b[:3]) * t[1]
+            n = n_b[:3] + (n_b[3:] - n_b[:3]) * t[2]
+            eta = eta_lo + (eta_hi - eta_lo) * rng.random()
+            c = _residual64(wi.astype(np.float64), wo.astype(np.float64), n.astype(np.float64), refract, eta)
+            if c is None:
+                continue
+            assert np.all(enc[:, 0] <= c) and np.all(c <= enc[:, 1]), (enc, c)
+            checked += 1
+        # the corners too
+        for corner in range(8):
+            pick = [(corner >> i) & 1 for i in range(3)]
+            wi = np.where(pick[0], wi_b[3:], wi_b[:3]).astype(np.float64)
+            wo = np.where(pick[1], wo_b[3:], wo_b[:3]).astype(np.float64)
+            n = np.where(pick[2], n_b[3:], n_b[:3]).astype(np.float64)
+            c = _residual64(wi, wo, n, refract, float(eta_lo))
+            if c is not None:
+                assert np.all(enc[:, 0] <= c) and np.all(c <= enc[:, 1])
+    assert checked > 1000
+
+
+def _eta_range(s):
+    lam = s.lambdas.astype(np.float32)
+    eta = np.float32(s.cauchy_a) + np.float32(s.cauchy_b) / (lam * lam)
+    return float(eta.min()), float(eta.max())
+
+
+CULL_SCENES = {
+    "c1_small": lambda: scenes.config1(width=48, photons=1 << 13),
+    "c2_small": lambda: scenes.config2(cells=24, width=64, photons=1 << 13, n_lambdas=3),
+    "pool": lambda: scenes.stack_scene(layer="pool", layer_cells=16, ior=1.33, seed=5, photons=1 << 12, width=40),
+    "mirror_slab": lambda: scenes.stack_scene(mirror_cells=10, layer="slab", layer_cells=14, ior=1.6, seed=6,
+                                              photons=1 << 12, width=40),
+}
+
+
+def scene_paths(name, far: bool):
+    """The scene's photon paths; with `far`, every path's receiver point is
+    moved to a far one (a (pixel, tuple) pair whose chain cannot close)."""
+    s = CULL_SCENES[name]()
+    nodes, root, *_ = oracle.lbvh(s.tris, bits=32)
+    recs, _ = oracle.photon_trace(s, nodes, root)
+    paths = oracle.lookup_paths(recs, s.points)
+    paths = paths[(paths["event"] >> 24) <= 3].copy()
+    if far:
+        n_pts = s.points.shape[0]
+        paths["pixel"] = (paths["pixel"] + n_pts // 2 + s.width // 2) % n_pts
+    return s, nodes, root, paths
+
+
+@pytest.mark.parametrize("far", [False, True])
+@pytest.mark.parametrize("name", sorted(CULL_SCENES))
+def test_culled_paths_never_contribute(name, far):
+    s, nodes, root, paths = scene_paths(name, far)
+    assert paths.shape[0] > 0
+    keep = oracle.interval_cull(s, paths, *_eta_range(s))
+    hs = oracle.HostScene(s, nodes, root)
+    res = oracle.solve_paths(hs, paths, threads=8)
+    L = s.lambdas.shape[0]
+    status = res["status"].reshape(-1, L)
+    culled = keep == 0
+    # conservative: no culled path has a contributing wavelength
+    assert not np.any(status[culled] == CA.PATH_CONTRIBUTES)
+    print(f"{name} far={far}: {paths.shape[0]} paths, culled {culled.sum()} ({culled.mean():.1%}); "
+          f"paths without a contributing wavelength {np.mean(~np.any(status == 2, axis=1)):.1%}")
+    if far and name != "mirror_slab":
+        # pairs that cannot close are culled (photon-made pairs sit next to their
+        # tuple's caustic, where the enclosures over whole triangles and their
+        # shading-normal boxes are too wide to exclude them; the three-vertex
+        # mirror+slab chains stay too wide even for far pairs)
+        assert culled.mean() > 0.05
+
+
+def test_planted_non_conservative_culling_is_caught():
+    """The check above detects a culling that is not conservative: culling a
+    path from every contributing tuple (what a dropped term could do) trips it."""
+    s = CULL_SCENES["c2_small"]()
+    nodes, root, *_ = oracle.lbvh(s.tris, bits=32)
+    recs, _ = oracle.photon_trace(s, nodes, root)
+    paths = oracle.lookup_paths(recs, s.points)
+    paths = paths[(paths["event"] >> 24) <= 3][:2000]
+    hs = oracle.HostScene(s, nodes, root)
+    status = oracle.solve_paths(hs, paths, threads=8)["status"].reshape(-1, s.lambdas.shape[0])
+    bad_keep = np.ones(paths.shape[0], dtype=np.uint8)
+    bad_keep[np.nonzero(np.any(status == 2, axis=1))[0][:1]] = 0
+    assert np.any(status[bad_keep == 0] == CA.PATH_CONTRIBUTES)
