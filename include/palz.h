@@ -298,6 +298,34 @@ int palz_solve_specular(int K, uint32_t event, const palz_scene_desc* scene, con
 int palz_interval_cull(const palz_scene_desc* d_scenes, uint32_t n_scenes, const palz_path* d_paths,
                        uint32_t n_paths, const uint32_t* d_count, uint8_t* d_admissible, void* stream);
 
+/* ---- adjoint of receiver irradiance (no reference counterpart) -----------
+ * Heights of a heightfield as parameters: a triangle's vertex c has height
+ * parameter tri_vertex[3 t + c] (-1: the triangle is not part of the
+ * heightfield); vertex y = height, and the vertex's shading normal is
+ * normalize(-dh/dx, 1, -dh/dz) with the derivatives taken as central (one-sided
+ * at the border) differences: normal_nb[4 h + j] names the j-th height the
+ * normal of parameter h depends on (-1: none) and normal_coef[8 h + 2 j + 0/1]
+ * its coefficients in dh/dx and dh/dz.  All three arrays are device memory. */
+typedef struct {
+    const int32_t* tri_vertex;
+    const int32_t* normal_nb;
+    const float* normal_coef;
+    uint32_t n_params;
+    uint32_t pad_;
+} palz_height_param;
+
+/* d_grad[h] += sum over contributing items of d_weight[l * n_points + pixel]
+ * * d(irradiance contribution)/d(height h), through the converged Newton
+ * solution by the implicit-function theorem (FP64; the tuple set, visibility
+ * and validity of the forward solve are held).  d_weight is dL/dI of the
+ * receiver image (n_lambdas x n_points).  With d_results (the forward solve's
+ * per-item records, d_paths in the same order) the contributing items and
+ * their FP32 solutions are read; with d_results NULL the forward solve is
+ * recomputed per item instead of stored. */
+int palz_adjoint_irradiance(const palz_scene_desc* scene, const palz_height_param* hp, const palz_path* d_paths,
+                            uint32_t n_paths, const palz_path_result* d_results, const float* d_weight,
+                            double* d_grad, void* stream);
+
 /* ---- batched persistent solver (no reference counterpart) ---------------
  * Replaces the reference's per-event-descriptor launch loop (pk.h:1480-1586,
  * one lookupFlatten + blocking count read + solveSpecularPath_K launch per

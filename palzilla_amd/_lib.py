@@ -66,6 +66,8 @@ _SIGS = {
                                       C.c_void_p, C.c_void_p]),
     "palz_interval_cull": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                      C.c_void_p]),
+    "palz_adjoint_irradiance": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p]),
     "palz_batched_create": (C.c_int, [c_void_pp]),
     "palz_batched_destroy": (C.c_int, [C.c_void_p]),
     "palz_batched_prepare": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -386,3 +386,28 @@ def interval_cull(d_scenes: DeviceBuffer, n_scenes: int, d_paths: DeviceBuffer |
     call("palz_interval_cull", C.c_void_p(d_scenes.ptr), C.c_uint32(n_scenes), C.c_void_p(ptr), C.c_uint32(n_paths),
          C.c_void_p(None), C.c_void_p(d_out.ptr), _sp(stream))
     return d_out.download(np.uint8, n_paths, stream)
+
+
+class HeightParamStruct(C.Structure):
+    _fields_ = [("tri_vertex", C.c_void_p), ("normal_nb", C.c_void_p), ("normal_coef", C.c_void_p),
+                ("n_params", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+class HeightParams:
+    """Device copy of a heightfield parameterisation (scenes.height_param_tables)."""
+
+    def __init__(self, tri_vertex: np.ndarray, normal_nb: np.ndarray, normal_coef: np.ndarray):
+        self.d_tv = DeviceBuffer.from_array(np.ascontiguousarray(tri_vertex, dtype=np.int32))
+        self.d_nb = DeviceBuffer.from_array(np.ascontiguousarray(normal_nb, dtype=np.int32))
+        self.d_coef = DeviceBuffer.from_array(np.ascontiguousarray(normal_coef, dtype=np.float32))
+        self.n_params = normal_nb.shape[0]
+        self.struct = HeightParamStruct(self.d_tv.ptr, self.d_nb.ptr, self.d_coef.ptr, self.n_params, 0)
+
+
+def adjoint_irradiance(g: "GpuScene", hp: HeightParams, d_paths: int, n_paths: int, d_results: int | None,
+                       d_weight: DeviceBuffer, d_grad: DeviceBuffer, stream=None) -> None:
+    """palz_adjoint_irradiance: d_grad (n_params doubles) += dL/dh for the
+    items of d_paths given dL/dI in d_weight (n_lambdas x n_points floats)."""
+    d = g.desc()
+    call("palz_adjoint_irradiance", C.byref(d), C.byref(hp.struct), C.c_void_p(d_paths), C.c_uint32(n_paths),
+         C.c_void_p(d_results), C.c_void_p(d_weight.ptr), C.c_void_p(d_grad.ptr), _sp(stream))

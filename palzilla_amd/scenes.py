@@ -228,6 +228,78 @@ def config2(seed: int = 2, cells: int = 256, width: int = 1024, photons: int = 1
                         info={"seed": seed, "cells": cells})
 
 
+def slab_heights(cells: int, seed: int, y0: float = 1.0, amplitude: float = 0.03) -> np.ndarray:
+    """Vertex heights (cells+1)^2 of a wavy slab top: y0 + 6 seeded sinusoids."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    xs = np.linspace(-1.0, 1.0, cells + 1)
+    X, Z = np.meshgrid(xs, xs, indexing="ij")
+    h, _, _ = _height(_sines(rng, 6, amplitude), X, Z)
+    return (y0 + h).reshape(-1)
+
+
+def config5(heights: np.ndarray, cells: int = 128, width: int = 512, photons: int = 1 << 21,
+            ior: float = 1.5) -> CausticScene:
+    """Adjoint workload: a glass slab whose top is a cells x cells heightfield
+    with the given vertex heights ((cells+1)^2 parameters; shading normals from
+    central differences of the heights), flat bottom at y = 0.5, IOR 1.5 at one
+    wavelength, light (0, 3, 0), receiver y = 0 over [-1.25, 1.25]^2 at width^2.
+    The heightfield's triangles come first (2 per cell, cell-major)."""
+    t, n, _ = heightfield(cells, 1.0, [], heights=np.asarray(heights, dtype=np.float64))
+    tb, nb = quad(0.5, (-1.0, 1.0, -1.0, 1.0), -1.0)
+    tr, nr = quad(0.0, (-3.0, 3.0, -3.0, 3.0), 1.0)
+    tris, att = _assemble([(t, n, T.MATERIAL_DIELECTRIC), (tb, nb, T.MATERIAL_DIELECTRIC),
+                           (tr, nr, T.MATERIAL_DIFFUSE)])
+    ext = (-1.25, 1.25, -1.25, 1.25)
+    return CausticScene(tris=tris, attribs=att, p_light=(0.0, 3.0, 0.0), points=receiver_points(width, width, ext),
+                        width=width, height=width, extent=ext, cauchy_a=ior, cauchy_b=0.0,
+                        lambdas=np.array([550.0], dtype=np.float32),
+                        photons_per_triangle=max(1, photons // t.shape[0]), eta_min=ior, eta_max=ior,
+                        name=f"C5 slab {cells}^2 TT, {width}^2, {photons} photons",
+                        info={"cells": cells, "heightfield_tris": t.shape[0]})
+
+
+def height_param_tables(cells: int, n_tris: int):
+    """The parameterisation of config5's heights for the adjoint: per
+    triangle its vertices' height indices (-1 outside the heightfield), per
+    height the 4 heights its normal's central differences use (-1: none) and
+    their coefficients in (dh/dx, dh/dz)."""
+    m = cells + 1
+    idx = np.arange(m * m, dtype=np.int32).reshape(m, m)
+    a = idx[:-1, :-1].reshape(-1)
+    b = idx[1:, :-1].reshape(-1)
+    c = idx[1:, 1:].reshape(-1)
+    d = idx[:-1, 1:].reshape(-1)
+    tv = np.full((n_tris, 3), -1, dtype=np.int32)
+    hf = np.stack([np.stack([a, b, c], axis=1), np.stack([a, c, d], axis=1)], axis=1).reshape(-1, 3)
+    tv[:hf.shape[0]] = hf
+    dx = 2.0 / cells
+    nb = np.full((m * m, 4), -1, dtype=np.int32)
+    coef = np.zeros((m * m, 8), dtype=np.float32)
+    for i in range(m):
+        for j in range(m):
+            h = i * m + j
+            # np.gradient: central inside, one-sided at the border (edge_order 1)
+            if 0 < i < cells:
+                xs = [(i - 1, -0.5 / dx), (i + 1, 0.5 / dx)]
+            elif i == 0:
+                xs = [(0, -1.0 / dx), (1, 1.0 / dx)]
+            else:
+                xs = [(cells - 1, -1.0 / dx), (cells, 1.0 / dx)]
+            if 0 < j < cells:
+                zs = [(j - 1, -0.5 / dx), (j + 1, 0.5 / dx)]
+            elif j == 0:
+                zs = [(0, -1.0 / dx), (1, 1.0 / dx)]
+            else:
+                zs = [(cells - 1, -1.0 / dx), (cells, 1.0 / dx)]
+            for e, (ii, cx) in enumerate(xs):
+                nb[h, e] = ii * m + j
+                coef[h, 2 * e] = cx
+            for e, (jj, cz) in enumerate(zs):
+                nb[h, 2 + e] = i * m + jj
+                coef[h, 2 * (2 + e) + 1] = cz
+    return tv, nb, coef
+
+
 def config4_scenes(n_scenes: int = 256, seed: int = 4, width: int = 64, photon_exp=(14, 20),
                    cell_exp=(4, 8)):
     """Batched workload: n_scenes stack scenes, heightfields of round(2^U[4,8])
