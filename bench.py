@@ -479,6 +479,81 @@ def bench_c4(args) -> dict:
 
 
 # ---------------------------------------------------------------------------------------------------
+# adjoint: d(L2 image loss)/d(heights) of a 128^2-cell refractive heightfield
+
+def bench_c5(args) -> dict:
+    from palzilla_amd import caustics as CA
+    from palzilla_amd import scenes
+    from palzilla_amd.device import DeviceBuffer, Event, Stream
+
+    cells, width = 128, 512
+    st = Stream()
+    h = scenes.slab_heights(cells, 5)
+
+    def forward(heights, g_prev=None):
+        s = scenes.config5(heights, cells=cells, width=width)
+        g = CA.GpuScene(s, stream=st)
+        return s, g
+
+    # target image from a second seed
+    s_t, g_t = forward(scenes.slab_heights(cells, 6))
+    g_t.trace()
+    g_t.build_table()
+    _, n_t = g_t.lookup(0, capacity=1)
+    d_pt, n_t = g_t.lookup(0, capacity=n_t)
+    bt = CA.BatchedSolver(g_t.sorter, stream=st)
+    bt.prepare(d_pt, n_t)
+    g_t.clear_image()
+    bt.solve(CA.upload_scene_descs([g_t.desc()], st), 1, 1, CA.REFINE_VERTICES, None)
+    target = g_t.image().reshape(-1)
+
+    s, g = forward(h)
+    tv, nb, coef = scenes.height_param_tables(cells, s.n_tris)
+    hp = CA.HeightParams(tv, nb, coef)
+    e = [Event() for _ in range(4)]
+    ms_fwd, ms_bwd = [], []
+    # sizes of the path list, so that the timed steps allocate nothing
+    g.trace()
+    g.build_table()
+    _, cap = g.lookup(0, capacity=1)
+    cap = int(cap * 1.1) + 1024
+    d_paths = DeviceBuffer(cap * CA.PATH.itemsize)
+    bs = CA.BatchedSolver(g.sorter, stream=st)
+    d_scenes = CA.upload_scene_descs([g.desc()], st)
+    d_res = DeviceBuffer(cap * CA.PATH_RESULT.itemsize)
+    d_g = DeviceBuffer(hp.n_params * 8)
+    for step in range(args.warmup + args.steps):
+        e[0].record(st)
+        g.trace()
+        g.build_table()
+        d_paths, n = g.lookup(0, capacity=cap, d_paths=d_paths)
+        bs.prepare(d_paths, min(n, cap))
+        g.clear_image()
+        bs.solve(d_scenes, 1, 1, CA.REFINE_VERTICES, d_res)
+        e[1].record(st)
+        img = g.image().reshape(-1)
+        weight = (2.0 * (img - target)).astype(np.float32)
+        d_w = DeviceBuffer.from_array(weight, st)
+        d_g.zero(st)
+        _, p_sorted, _ = bs.pointers()
+        e[2].record(st)
+        CA.adjoint_irradiance(g, hp, p_sorted, min(n, cap), d_res.ptr, d_w, d_g, st)
+        e[3].record(st)
+        st.sync()
+        if step >= args.warmup:
+            ms_fwd.append(e[0].elapsed_ms(e[1]))
+            ms_bwd.append(e[2].elapsed_ms(e[3]))
+    grad = d_g.download(np.float64, hp.n_params, st)
+    loss = float(np.sum((img.astype(np.float64) - target) ** 2))
+    fwd, bwd = float(np.median(ms_fwd)), float(np.median(ms_bwd))
+    return {"workload": "adjoint: 128^2-cell refractive heightfield (16,641 vertex heights), IOR 1.5, "
+                        "512^2 receiver, L2 loss to a second seed's image",
+            "paths": int(n), "params": int(hp.n_params), "loss": loss, "grad_norm": float(np.linalg.norm(grad)),
+            "forward_ms": round(fwd, 3), "backward_ms": round(bwd, 3), "backward_over_forward": round(bwd / fwd, 3),
+            "forward_includes": "photon trace, tuple table, flatten, batched solve (FP32 + FP64 step), image"}
+
+
+# ---------------------------------------------------------------------------------------------------
 # LBVH + onesweep
 
 def bench_lbvh(args) -> dict:
@@ -543,7 +618,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "lbvh"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5", "lbvh"])
     ap.add_argument("--no-extras", action="store_true", help="c2 only, without the c4 and lbvh sub-results")
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--cpu-sample", type=int, default=1 << 22, help="LBVH CPU baseline triangles")
@@ -562,7 +637,13 @@ def main() -> int:
         out = bench_c2(args)
         if not args.no_extras:
             out["c4"] = bench_c4(args)
+            out["c5"] = bench_c5(args)
             out["lbvh"] = bench_lbvh(args)
+    elif args.workload == "c5":
+        r = bench_c5(args)
+        out = {"metric": METRIC, "value": round(r["params"] / (r["backward_ms"] * 1e-3), 1),
+               "unit": "height gradients/s", "ms_per_step": r["forward_ms"] + r["backward_ms"], "dtype": "f32+f64",
+               "config": {"workload": r["workload"]}, "c5": r}
     elif args.workload == "c4":
         r = bench_c4(args)
         out = {"metric": METRIC, "value": r["batched_paths_per_s"], "unit": "paths/s",
