@@ -12,6 +12,7 @@
 //     the escape link is still written into `context`;
 //   * the build launches one thread per leaf (the reference launches n-1
 //     threads, minimum_lbvh.h:875, and skips the last leaf when n % 256 == 1).
+#include "prims.hpp"
 #include "sort.hpp"
 
 #include <algorithm>
@@ -231,60 +232,204 @@ __device__ __forceinline__ void load_box_coherent(palz_aabb* src, v3* lo, v3* hi
     *hi = mk(__uint_as_float((uint32_t)(b >> 32)), __uint_as_float((uint32_t)c), __uint_as_float((uint32_t)(c >> 32)));
 }
 
-// build + build_lbvh (minimum_lbvh.cu:104-121, minimum_lbvh.h:293-370).
-__global__ void __launch_bounds__(256) k_build(const palz_triangle* __restrict__ tris, uint32_t n,
-                                               const uint8_t* __restrict__ deltas,
-                                               const uint64_t* __restrict__ words,
-                                               uint32_t* __restrict__ arrival,
-                                               palz_internal_node* nodes,
-                                               palz_node_index* __restrict__ root) {
-    const uint32_t leaf = blockIdx.x * blockDim.x + threadIdx.x;
-    if (leaf >= n) return;
+// build + build_lbvh (minimum_lbvh.cu:104-121, minimum_lbvh.h:293-370), in
+// levels of workgroup-local climbs.
+//
+// Level 1 climbs from the sorted leaves, one workgroup per chunk of
+// kBuildChunk consecutive leaves.  Node p (the split between leaves p and
+// p+1) is "local" when both its children lie in the chunk, i.e. its leaf range
+// [lo, hi] does; that range is fixed by the deltas alone (lo - 1 = nearest
+// split on the left with delta >= delta[p], hi = nearest split on the right
+// with delta > delta[p]), so prefix / suffix maxima of the chunk's deltas in
+// LDS decide it before the climb.  Local nodes hand their first child over in
+// LDS, and their records are written out whole at the end of the chunk.  A
+// climb that reaches a non-local node stops there: its subtree root becomes an
+// item of the next level.  Those items, in leaf order, are the leaves of the
+// tree above (the non-local nodes split exactly at the boundaries between
+// them, with the same deltas), so the next level runs the same climb on them,
+// until one chunk holds every item.  No level needs device-wide coherence,
+// and both children of a node always agree on its level; the boxes are exact
+// min/max, so the tree is the bit-identical result of the one-thread-per-leaf
+// climb.
+constexpr int kBuildChunk = 512;
+
+// A subtree root handed to the next level: its node, leaf range and box.
+struct BuildItem {
+    uint32_t node, lo, hi, pad_;
+    float box[6];
+};
+
+template <bool kLeaves>
+__global__ void __launch_bounds__(kBuildChunk)
+    k_build(const palz_triangle* __restrict__ tris, const uint64_t* __restrict__ words,
+            const BuildItem* __restrict__ items, uint32_t m, uint32_t n, const uint8_t* __restrict__ deltas,
+            palz_internal_node* __restrict__ nodes, palz_node_index* __restrict__ root,
+            BuildItem* __restrict__ staging, uint32_t* __restrict__ chunk_count,
+            uint32_t* __restrict__ chunk_bits) {
+    __shared__ uint32_t s_delta[kBuildChunk];  // delta of the split after each of the chunk's items
+    __shared__ uint32_t s_pre[kBuildChunk];    // max of s_delta up to t
+    __shared__ uint32_t s_suf[kBuildChunk];    // max of s_delta from t on
+    __shared__ uint8_t s_local[kBuildChunk];
+    __shared__ uint32_t s_flag[kBuildChunk];
+    __shared__ uint32_t s_up[kBuildChunk / 32];             // chunk items that go up a level
+    __shared__ palz_internal_node s_node[kBuildChunk];      // records of the local nodes
     const uint32_t n_int = n - 1u;
-    const uint32_t tri_index = (uint32_t)(words[leaf] >> 32);
-
-    uint32_t lo = leaf, hi = leaf;
-    palz_node_index node = tri_index | PALZ_NODE_LEAF_BIT;
-    const palz_triangle t = tris[tri_index];
-    v3 blo = mk(t.vs[0]), bhi = mk(t.vs[0]);
-    blo = cmin(blo, mk(t.vs[1]));
-    bhi = cmax(bhi, mk(t.vs[1]));
-    blo = cmin(blo, mk(t.vs[2]));
-    bhi = cmax(bhi, mk(t.vs[2]));
-
-    while (hi - lo < n_int) {
-        const uint32_t dl = lo > 0u ? (uint32_t)deltas[lo - 1u] : 0xFFFFFFFFu;
-        const uint32_t dr = hi < n_int ? (uint32_t)deltas[hi] : 0xFFFFFFFFu;
-        const int go_left = dl < dr ? 1 : 0;
-        const uint32_t parent = go_left ? lo - 1u : hi;
-
-        palz_internal_node* pn = nodes + parent;
-        pn->children[go_left] = node;
-        store_box_coherent(&pn->aabbs[go_left], blo, bhi);
-        if (!(node & PALZ_NODE_LEAF_BIT)) nodes[node].parent = parent;
-
-        // The first of the two children to arrive stops; the second merges.
-        // The range bound travels in the exchanged value itself.
-        const uint32_t mine = go_left ? hi : lo;
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0);  // the coherent box stores have completed
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        const uint32_t other = __hip_atomic_exchange(arrival + parent, mine, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (other == kNoArrival) return;
-
-        lo = min(lo, other);
-        hi = max(hi, other);
-        pn->context = hi < n_int ? hi : 0xFFFFFFFFu;
-        node = parent;
-        v3 slo, shi;
-        load_box_coherent(&pn->aabbs[go_left ^ 1], &slo, &shi);
-        blo = cmin(blo, slo);
-        bhi = cmax(bhi, shi);
+    const uint32_t m_int = m - 1u;  // item-level splits
+    const uint32_t c0 = blockIdx.x * kBuildChunk;
+    const uint32_t c_last = min(c0 + (uint32_t)kBuildChunk, m) - 1u;  // last item of the chunk
+    const uint32_t cnt = c_last - c0;                                   // item splits c0 .. c_last-1
+    const uint32_t t = threadIdx.x;
+    auto item_hi = [&](uint32_t j) -> uint32_t { return kLeaves ? j : items[j].hi; };
+    auto split_delta = [&](uint32_t j) -> uint32_t { return (uint32_t)deltas[item_hi(j)]; };
+    uint32_t pm = t < cnt ? split_delta(c0 + t) : 0u, sm = pm;
+    s_delta[t] = pm;
+    s_pre[t] = pm;
+    s_suf[t] = sm;
+    s_flag[t] = kNoArrival;
+    if (t < kBuildChunk / 32) s_up[t] = 0u;
+    __syncthreads();
+    for (uint32_t off = 1; off < (uint32_t)kBuildChunk; off <<= 1) {
+        const uint32_t x = t >= off ? s_pre[t - off] : 0u;
+        const uint32_t y = t + off < cnt ? s_suf[t + off] : 0u;
+        __syncthreads();
+        pm = max(pm, x);
+        sm = max(sm, y);
+        s_pre[t] = pm;
+        s_suf[t] = sm;
+        __syncthreads();
     }
-    *root = node;
-    if (!(node & PALZ_NODE_LEAF_BIT)) nodes[node].parent = PALZ_NODE_INVALID;
+    if (t < cnt) {
+        const uint32_t dp = s_delta[t];
+        const bool left = (t > 0u && s_pre[t - 1u] >= dp) || c0 == 0u || split_delta(c0 - 1u) >= dp;
+        const bool right = (t + 1u < cnt && s_suf[t + 1u] > dp) || c_last == m_int || split_delta(c_last) > dp;
+        s_local[t] = (left && right) ? 1 : 0;
+    }
+    __syncthreads();
+
+    const uint32_t it = c0 + t;
+    if (it < m) {
+        uint32_t a = it, bb = it;  // item range
+        uint32_t lo, hi;           // leaf range
+        palz_node_index node;
+        v3 blo, bhi;
+        if (kLeaves) {
+            lo = hi = it;
+            const uint32_t tri_index = (uint32_t)(words[it] >> 32);
+            node = tri_index | PALZ_NODE_LEAF_BIT;
+            const palz_triangle tr = tris[tri_index];
+            blo = mk(tr.vs[0]);
+            bhi = mk(tr.vs[0]);
+            blo = cmin(blo, mk(tr.vs[1]));
+            bhi = cmax(bhi, mk(tr.vs[1]));
+            blo = cmin(blo, mk(tr.vs[2]));
+            bhi = cmax(bhi, mk(tr.vs[2]));
+        } else {
+            const BuildItem in = items[it];
+            lo = in.lo;
+            hi = in.hi;
+            node = in.node;
+            blo = mk(in.box[0], in.box[1], in.box[2]);
+            bhi = mk(in.box[3], in.box[4], in.box[5]);
+        }
+        auto delta_at = [&](uint32_t j) -> uint32_t {  // item split j
+            return j - c0 < cnt ? s_delta[j - c0] : split_delta(j);
+        };
+        bool go_up = false;
+        while (bb - a < m_int) {
+            const uint32_t dl = a > 0u ? delta_at(a - 1u) : 0xFFFFFFFFu;
+            const uint32_t dr = bb < m_int ? delta_at(bb) : 0xFFFFFFFFu;
+            const int go_left = dl < dr ? 1 : 0;
+            const uint32_t parent = go_left ? a - 1u : bb;  // item split
+            if (!(parent - c0 < cnt && s_local[parent - c0])) {
+                go_up = true;
+                break;
+            }
+            palz_internal_node& rec = s_node[parent - c0];
+            rec.children[go_left] = node;
+            rec.aabbs[go_left].lower = {blo.x, blo.y, blo.z};
+            rec.aabbs[go_left].upper = {bhi.x, bhi.y, bhi.z};
+            const uint32_t mine = go_left ? bb : a;
+            const uint32_t other = __hip_atomic_exchange(&s_flag[parent - c0], mine, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (other == kNoArrival) break;
+            const palz_aabb sb = rec.aabbs[go_left ^ 1];
+            if (go_left) {
+                a = other;
+                lo = kLeaves ? other : items[other].lo;
+            } else {
+                bb = other;
+                hi = kLeaves ? other : items[other].hi;
+            }
+            uint32_t grand = PALZ_NODE_INVALID;
+            if (hi - lo < n_int) {
+                const uint32_t gl = lo > 0u ? (uint32_t)deltas[lo - 1u] : 0xFFFFFFFFu;
+                const uint32_t gr = hi < n_int ? (uint32_t)deltas[hi] : 0xFFFFFFFFu;
+                grand = gl < gr ? lo - 1u : hi;
+            }
+            rec.parent = grand;
+            rec.context = hi < n_int ? hi : 0xFFFFFFFFu;
+            node = kLeaves ? parent : item_hi(parent);  // the split's leaf-level index
+            blo = cmin(blo, mk(sb.lower));
+            bhi = cmax(bhi, mk(sb.upper));
+        }
+        if (bb - a == m_int) {
+            *root = node;
+        } else if (go_up) {
+            // this subtree root is an item of the next level, staged at its first item
+            BuildItem up;
+            up.node = node;
+            up.lo = lo;
+            up.hi = hi;
+            up.box[0] = blo.x;
+            up.box[1] = blo.y;
+            up.box[2] = blo.z;
+            up.box[3] = bhi.x;
+            up.box[4] = bhi.y;
+            up.box[5] = bhi.z;
+            up.pad_ = 0u;
+            staging[a] = up;
+            atomicOr(&s_up[(a - c0) >> 5], 1u << ((a - c0) & 31u));
+        }
+    }
+    __syncthreads();
+    // the chunk's local records, written out whole
+    for (uint32_t i = t; i < cnt; i += blockDim.x) {
+        if (!s_local[i]) continue;
+        const float4* src = reinterpret_cast<const float4*>(&s_node[i]);
+        float4* dst = reinterpret_cast<float4*>(nodes + item_hi(c0 + i));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dst[q] = src[q];
+    }
+    // which of the chunk's positions hold an item of the next level
+    if (staging) {
+        if (t < kBuildChunk / 32) chunk_bits[blockIdx.x * (kBuildChunk / 32) + t] = s_up[t];
+        if (t == 0) {
+            uint32_t total = 0;
+            for (uint32_t w = 0; w < kBuildChunk / 32; ++w) total += __popc(s_up[w]);
+            chunk_count[blockIdx.x] = total;
+        }
+    }
+}
+
+// Compacts the staged items of each chunk (staging[c * kBuildChunk + j] for
+// the set bits j of its bitmap) to out[offset[c] + rank]: the next level's
+// items in leaf order.  One workgroup of kBuildChunk threads per chunk.
+__global__ void __launch_bounds__(kBuildChunk) k_build_gather(const BuildItem* __restrict__ staging,
+                                                              const uint32_t* __restrict__ bits,
+                                                              const uint32_t* __restrict__ offset,
+                                                              uint32_t n_chunks, BuildItem* __restrict__ out) {
+    __shared__ uint32_t s_bits[kBuildChunk / 32];
+    for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        __syncthreads();
+        if (threadIdx.x < kBuildChunk / 32) s_bits[threadIdx.x] = bits[c * (kBuildChunk / 32) + threadIdx.x];
+        __syncthreads();
+        const uint32_t t = threadIdx.x, word = t >> 5, bit = 1u << (t & 31u);
+        if (s_bits[word] & bit) {
+            uint32_t rank = __popc(s_bits[word] & (bit - 1u));
+            for (uint32_t w = 0; w < word; ++w) rank += __popc(s_bits[w]);
+            out[offset[c] + rank] = staging[(size_t)c * kBuildChunk + t];
+        }
+    }
 }
 
 }  // namespace
@@ -299,6 +444,11 @@ struct palz_lbvh {
     uint64_t* tmp = nullptr;
     uint8_t* deltas = nullptr;
     uint32_t* arrival = nullptr;
+    palz::BuildItem* staging = nullptr;  // per chunk, the items that go up a level
+    palz::BuildItem* items[2] = {};      // the items of a level (ping-pong)
+    uint32_t items_cap = 0;
+    uint32_t* chunk_count = nullptr;     // [chunks] counts, [chunks] offsets, scan sums, total
+    uint32_t* chunk_bits = nullptr;      // per chunk, the positions staged for the next level
     uint32_t capacity = 0;
     int n = 0;
     // Optional stage timing: events after each stage of the last build.
@@ -314,6 +464,10 @@ struct palz_lbvh {
         PALZ_HIP(hipMalloc(&tmp, sizeof(uint64_t) * want));
         PALZ_HIP(hipMalloc(&deltas, std::max(4u, want)));
         PALZ_HIP(hipMalloc(&arrival, sizeof(uint32_t) * std::max(4u, want)));
+        const uint32_t chunks = palz::div_up(want, (uint32_t)palz::kBuildChunk);
+        PALZ_HIP(hipMalloc(&staging, sizeof(palz::BuildItem) * (size_t)chunks * palz::kBuildChunk));
+        PALZ_HIP(hipMalloc(&chunk_count, sizeof(uint32_t) * (2u * chunks + palz::scan_scratch_entries(chunks) + 4u)));
+        PALZ_HIP(hipMalloc(&chunk_bits, sizeof(uint32_t) * chunks * (palz::kBuildChunk / 32)));
         capacity = want;
         return PALZ_OK;
     }
@@ -323,6 +477,16 @@ struct palz_lbvh {
         if (tmp) (void)hipFree(tmp);
         if (deltas) (void)hipFree(deltas);
         if (arrival) (void)hipFree(arrival);
+        if (staging) (void)hipFree(staging);
+        if (chunk_count) (void)hipFree(chunk_count);
+        if (chunk_bits) (void)hipFree(chunk_bits);
+        chunk_bits = nullptr;
+        for (auto& p : items)
+            if (p) (void)hipFree(p);
+        staging = nullptr;
+        chunk_count = nullptr;
+        items[0] = items[1] = nullptr;
+        items_cap = 0;
         nodes = nullptr;
         words = tmp = nullptr;
         deltas = nullptr;
@@ -403,8 +567,46 @@ int palz_lbvh_build(palz_lbvh* b, const palz_triangle* d_tris, int n, uint32_t b
         k_deltas<<<div_up(quads, 256u), 256, 0, st>>>(b->words, un, b->deltas, b->arrival);
     }
     if (ev) PALZ_HIP(hipEventRecord(ev[7], st));
-    k_build<<<div_up(un, 256u), 256, 0, st>>>(d_tris, un, b->deltas, b->words, b->arrival, b->nodes,
-                                              b->root);
+    // level by level, until one chunk holds every item
+    {
+        uint32_t m = un;
+        int cur = -1;  // level 1 reads the leaves
+        for (int level = 0;; ++level) {
+            const uint32_t chunks = div_up(m, (uint32_t)kBuildChunk);
+            BuildItem* stage = chunks > 1u ? b->staging : nullptr;
+            if (cur < 0)
+                k_build<true><<<chunks, kBuildChunk, 0, st>>>(d_tris, b->words, nullptr, m, un, b->deltas, b->nodes,
+                                                              b->root, stage, b->chunk_count, b->chunk_bits);
+            else
+                k_build<false><<<chunks, kBuildChunk, 0, st>>>(nullptr, nullptr, b->items[cur], m, un, b->deltas,
+                                                               b->nodes, b->root, stage, b->chunk_count,
+                                                               b->chunk_bits);
+            PALZ_LAUNCH_CHECK();
+            if (chunks == 1u) break;
+            if (level > 64) return fail(PALZ_ERR_HIP, "palz_lbvh_build: the level-wise build does not converge");
+            uint32_t* counts = b->chunk_count;
+            uint32_t* offsets = counts + chunks;
+            uint32_t* sums = offsets + chunks;
+            uint32_t* total = sums + scan_scratch_entries(chunks);
+            launch_exclusive_scan(counts, offsets, chunks, nullptr, total, sums, st);
+            uint32_t next = 0;
+            PALZ_HIP(hipMemcpyAsync(&next, total, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            PALZ_HIP(hipStreamSynchronize(st));
+            if (next > b->items_cap) {
+                for (auto& p : b->items)
+                    if (p) PALZ_HIP(hipFree(p));
+                const uint32_t cap = std::max(next, b->items_cap * 2u);
+                PALZ_HIP(hipMalloc(&b->items[0], sizeof(BuildItem) * cap));
+                PALZ_HIP(hipMalloc(&b->items[1], sizeof(BuildItem) * cap));
+                b->items_cap = cap;
+            }
+            const int out = cur < 0 ? 0 : cur ^ 1;
+            k_build_gather<<<std::min(chunks, 8192u), kBuildChunk, 0, st>>>(b->staging, b->chunk_bits, offsets,
+                                                                             chunks, b->items[out]);
+            cur = out;
+            m = next;
+        }
+    }
     if (ev) PALZ_HIP(hipEventRecord(ev[8], st));
     PALZ_LAUNCH_CHECK();
     return PALZ_OK;
