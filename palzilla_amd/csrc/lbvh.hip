@@ -266,38 +266,43 @@ __global__ void __launch_bounds__(kBuildChunk)
             palz_internal_node* __restrict__ nodes, palz_node_index* __restrict__ root,
             BuildItem* __restrict__ staging, uint32_t* __restrict__ chunk_count,
             uint32_t* __restrict__ chunk_bits) {
-    __shared__ uint32_t s_delta[kBuildChunk];  // delta of the split after each of the chunk's items
-    __shared__ uint32_t s_pre[kBuildChunk];    // max of s_delta up to t
-    __shared__ uint32_t s_suf[kBuildChunk];    // max of s_delta from t on
+    __shared__ uint8_t s_delta[kBuildChunk];  // delta of the split after each of the chunk's items
+    __shared__ uint8_t s_pre[kBuildChunk];    // max of s_delta up to t
+    __shared__ uint8_t s_suf[kBuildChunk];    // max of s_delta from t on
     __shared__ uint8_t s_local[kBuildChunk];
     __shared__ uint32_t s_flag[kBuildChunk];
     __shared__ uint32_t s_up[kBuildChunk / 32];             // chunk items that go up a level
+    __shared__ uint32_t s_wave[2][kBuildChunk / 64];        // per-wave prefix / suffix maxima
     __shared__ palz_internal_node s_node[kBuildChunk];      // records of the local nodes
     const uint32_t n_int = n - 1u;
     const uint32_t m_int = m - 1u;  // item-level splits
     const uint32_t c0 = blockIdx.x * kBuildChunk;
     const uint32_t c_last = min(c0 + (uint32_t)kBuildChunk, m) - 1u;  // last item of the chunk
     const uint32_t cnt = c_last - c0;                                   // item splits c0 .. c_last-1
-    const uint32_t t = threadIdx.x;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     auto item_hi = [&](uint32_t j) -> uint32_t { return kLeaves ? j : items[j].hi; };
     auto split_delta = [&](uint32_t j) -> uint32_t { return (uint32_t)deltas[item_hi(j)]; };
-    uint32_t pm = t < cnt ? split_delta(c0 + t) : 0u, sm = pm;
-    s_delta[t] = pm;
-    s_pre[t] = pm;
-    s_suf[t] = sm;
+    const uint32_t d = t < cnt ? split_delta(c0 + t) : 0u;
+    s_delta[t] = (uint8_t)d;
     s_flag[t] = kNoArrival;
     if (t < kBuildChunk / 32) s_up[t] = 0u;
-    __syncthreads();
-    for (uint32_t off = 1; off < (uint32_t)kBuildChunk; off <<= 1) {
-        const uint32_t x = t >= off ? s_pre[t - off] : 0u;
-        const uint32_t y = t + off < cnt ? s_suf[t + off] : 0u;
-        __syncthreads();
-        pm = max(pm, x);
-        sm = max(sm, y);
-        s_pre[t] = pm;
-        s_suf[t] = sm;
-        __syncthreads();
+    // prefix and suffix maxima: within the wave by shuffles, then across waves
+    uint32_t pm = d, sm = d;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t x = (uint32_t)__shfl_up((int)pm, o, 64);
+        const uint32_t y = (uint32_t)__shfl_down((int)sm, o, 64);
+        if (lane >= (uint32_t)o) pm = max(pm, x);
+        if (lane + o < 64u) sm = max(sm, y);
     }
+    if (lane == 63u) s_wave[0][wave] = pm;
+    if (lane == 0u) s_wave[1][wave] = sm;
+    __syncthreads();
+    for (uint32_t w = 0; w < wave; ++w) pm = max(pm, s_wave[0][w]);
+    for (uint32_t w = wave + 1; w < kBuildChunk / 64; ++w) sm = max(sm, s_wave[1][w]);
+    s_pre[t] = (uint8_t)pm;
+    s_suf[t] = (uint8_t)sm;
+    __syncthreads();
     if (t < cnt) {
         const uint32_t dp = s_delta[t];
         const bool left = (t > 0u && s_pre[t - 1u] >= dp) || c0 == 0u || split_delta(c0 - 1u) >= dp;
