@@ -24,6 +24,26 @@ namespace {
 constexpr int kTriTile = 256;  // triangles per block in the staged kernels
 constexpr uint32_t kNoArrival = 0xFFFFFFFFu;
 
+constexpr int kBuildChunk = 512;  // sorted leaves per build workgroup
+constexpr int kChunkRoots = 64;   // staged subtree roots per chunk or group
+constexpr int kGroupChunks = 16;  // lower chunks per workgroup of the levels above
+constexpr int kGroupItems = kGroupChunks * kChunkRoots;
+constexpr uint32_t kDeviceChunk = 0xFFFFFFFFu;  // staged count of a chunk whose roots climbed device-wide
+
+// A finished subtree whose parent lies across chunks: its node, leaf range and box.
+struct alignas(16) SubtreeRoot {
+    uint32_t node, lo, hi, pad_;
+    float box[6];
+    uint32_t pad2_[2];
+};
+
+// Per-build control word: k_morton_hist bumps the epoch that tags this build's
+// arrival words, so they never need clearing.
+struct BuildCtl {
+    uint32_t epoch;
+    uint32_t pad_[3];
+};
+
 // Stage `count` (<= kTriTile) triangles starting at `first` into LDS.  With
 // `aligned` the block reads its 9216-byte window as float4s.
 __device__ __forceinline__ void stage_triangles(const palz_triangle* __restrict__ tris, uint32_t first,
@@ -132,9 +152,11 @@ __global__ void __launch_bounds__(kTriTile) k_morton_hist(const palz_triangle* _
                                                           const palz_aabb* __restrict__ scene,
                                                           uint32_t option, int bits,
                                                           uint64_t* __restrict__ words,
-                                                          uint32_t* __restrict__ hist) {
+                                                          uint32_t* __restrict__ hist,
+                                                          BuildCtl* __restrict__ ctl) {
     __shared__ float s_tri[kTriTile * 9];
     __shared__ uint32_t s_h[4 * 256];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch = ctl->epoch + 1u;  // the build's arrival epoch
     for (int i = threadIdx.x; i < 4 * 256; i += blockDim.x) s_h[i] = 0u;
     // Blocks loop over tiles so that each flushes its digit counts only once.
     const uint32_t ntiles = div_up(n, (uint32_t)kTriTile);
@@ -176,44 +198,14 @@ __global__ void __launch_bounds__(kTriTile) k_morton_hist(const palz_triangle* _
     }
 }
 
-// computeDeltas (minimum_lbvh.cu:94-102), four deltas per thread, plus the
-// arrival words of the build.
-__global__ void __launch_bounds__(256) k_deltas(const uint64_t* __restrict__ words, uint32_t n,
-                                                uint8_t* __restrict__ deltas,
-                                                uint32_t* __restrict__ arrival) {
-    const uint32_t m = n - 1u;  // number of deltas / internal nodes
-    const uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4u;
-    if (i0 >= m) return;
-    if (i0 + 4u <= m) {
-        uint32_t k[5];
-#pragma unroll
-        for (int j = 0; j < 5; ++j) k[j] = (uint32_t)words[i0 + j];
-        uint32_t packed = 0u;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) packed |= delta32(k[j], k[j + 1]) << (8 * j);
-        if ((i0 & 3u) == 0u && (reinterpret_cast<uintptr_t>(deltas) & 3u) == 0u) {
-            *reinterpret_cast<uint32_t*>(deltas + i0) = packed;
-        } else {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) deltas[i0 + j] = (uint8_t)(packed >> (8 * j));
-        }
-        *reinterpret_cast<uint4*>(arrival + i0) = make_uint4(kNoArrival, kNoArrival, kNoArrival, kNoArrival);
-    } else {
-        for (uint32_t i = i0; i < m; ++i) {
-            deltas[i] = (uint8_t)delta32((uint32_t)words[i], (uint32_t)words[i + 1]);
-            arrival[i] = kNoArrival;
-        }
-    }
-}
-
-// Box hand-over between the two children of a node.  On MI355X every XCD has
-// its own L2, and an agent-scope release/acquire fence (the reference's
-// __threadfence around atomicExch, minimum_lbvh.h:340-342) writes back and
-// invalidates the whole L2 (buffer_wbl2 / buffer_inv) — once per climb step.
-// Instead the box travels in three coherent 64-bit stores (sc1: written
-// through to the device-coherent level), a vmcnt wait orders them before the
-// arrival exchange, and the second arriver reads the box back with coherent
-// loads.  Nothing else in the node is read during the build.
+// Box hand-over between the two children of a node in the device-wide climb.
+// On MI355X every XCD has its own L2, and an agent-scope release/acquire fence
+// (the reference's __threadfence around atomicExch, minimum_lbvh.h:340-342)
+// writes back and invalidates the whole L2 — once per climb step.  Instead the
+// box travels in three coherent 64-bit stores (written through to the
+// device-coherent level), a vmcnt wait orders them before the arrival
+// exchange, and the second arriver reads the box back with coherent loads.
+// Nothing else in the node is read during the build.
 __device__ __forceinline__ uint64_t pack2(float a, float b) {
     return (uint64_t)__float_as_uint(a) | ((uint64_t)__float_as_uint(b) << 32);
 }
@@ -232,61 +224,175 @@ __device__ __forceinline__ void load_box_coherent(palz_aabb* src, v3* lo, v3* hi
     *hi = mk(__uint_as_float((uint32_t)(b >> 32)), __uint_as_float((uint32_t)c), __uint_as_float((uint32_t)(c >> 32)));
 }
 
-// build + build_lbvh (minimum_lbvh.cu:104-121, minimum_lbvh.h:293-370), in
-// levels of workgroup-local climbs.
-//
-// Level 1 climbs from the sorted leaves, one workgroup per chunk of
-// kBuildChunk consecutive leaves.  Node p (the split between leaves p and
-// p+1) is "local" when both its children lie in the chunk, i.e. its leaf range
-// [lo, hi] does; that range is fixed by the deltas alone (lo - 1 = nearest
-// split on the left with delta >= delta[p], hi = nearest split on the right
-// with delta > delta[p]), so prefix / suffix maxima of the chunk's deltas in
-// LDS decide it before the climb.  Local nodes hand their first child over in
-// LDS, and their records are written out whole at the end of the chunk.  A
-// climb that reaches a non-local node stops there: its subtree root becomes an
-// item of the next level.  Those items, in leaf order, are the leaves of the
-// tree above (the non-local nodes split exactly at the boundaries between
-// them, with the same deltas), so the next level runs the same climb on them,
-// until one chunk holds every item.  No level needs device-wide coherence,
-// and both children of a node always agree on its level; the boxes are exact
-// min/max, so the tree is the bit-identical result of the one-thread-per-leaf
-// climb.
-constexpr int kBuildChunk = 512;
+// The Morton key of sorted leaf j: the low half of its word.
+__device__ __forceinline__ uint32_t key_at(const uint64_t* __restrict__ words, uint32_t j) {
+    return reinterpret_cast<const uint32_t*>(words)[2u * j];
+}
 
-// A subtree root handed to the next level: its node, leaf range and box.
-struct BuildItem {
-    uint32_t node, lo, hi, pad_;
-    float box[6];
+// The device-wide climb (build_lbvh's loop, minimum_lbvh.h:293-370) from a
+// finished subtree (node, leaf range, box) whose parent lies across chunks.
+// Arrival words hold (epoch, far leaf end); a word from an earlier build reads
+// as "no arrival".  kKeys: range-end deltas come from the sorted keys (inside
+// the build kernel, where the chunk that writes them may not have run yet);
+// otherwise from the delta array.
+template <bool kKeys>
+__device__ void climb_device(palz_node_index node, uint32_t lo, uint32_t hi, v3 blo, v3 bhi, uint32_t n,
+                             const uint64_t* __restrict__ words, const uint8_t* __restrict__ deltas,
+                             uint64_t* __restrict__ arrival, palz_internal_node* nodes,
+                             palz_node_index* __restrict__ root, uint32_t ep) {
+    const uint32_t n_int = n - 1u;
+    auto split_delta = [&](uint32_t j) -> uint32_t {
+        return kKeys ? delta32(key_at(words, j), key_at(words, j + 1u)) : (uint32_t)deltas[j];
+    };
+    bool finished_here = false;  // `node` was completed by this climb (not by its chunk)
+    while (hi - lo < n_int) {
+        const uint32_t dl = lo > 0u ? split_delta(lo - 1u) : 0xFFFFFFFFu;
+        const uint32_t dr = hi < n_int ? split_delta(hi) : 0xFFFFFFFFu;
+        const int go_left = dl < dr ? 1 : 0;
+        const uint32_t parent = go_left ? lo - 1u : hi;
+        palz_internal_node* pn = nodes + parent;
+        pn->children[go_left] = node;
+        store_box_coherent(&pn->aabbs[go_left], blo, bhi);
+        if (finished_here) nodes[node].parent = parent;
+        // The first of the two children to arrive stops; the second merges.
+        // The range bound travels in the exchanged value itself.
+        const uint64_t mine = ((uint64_t)ep << 32) | (uint64_t)(go_left ? hi : lo);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);  // the coherent box stores have completed
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const uint64_t old = __hip_atomic_exchange(arrival + parent, mine, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if ((uint32_t)(old >> 32) != ep) return;
+        const uint32_t other = (uint32_t)old;
+        lo = min(lo, other);
+        hi = max(hi, other);
+        pn->context = hi < n_int ? hi : 0xFFFFFFFFu;
+        node = parent;
+        finished_here = true;
+        v3 slo, shi;
+        load_box_coherent(&pn->aabbs[go_left ^ 1], &slo, &shi);
+        blo = cmin(blo, slo);
+        bhi = cmax(bhi, shi);
+    }
+    *root = node;
+    if (finished_here) nodes[node].parent = PALZ_NODE_INVALID;
+}
+
+// The count a chunk (or group) reports for its finished subtree roots:
+// kDeviceChunk once they are more than its kChunkRoots slots.
+__device__ __forceinline__ uint32_t staged_total(const uint32_t* s_up, uint32_t words) {
+    uint32_t count = 0u;
+    for (uint32_t w = 0; w < words; ++w) count += __popc(s_up[w]);
+    return count > (uint32_t)kChunkRoots ? kDeviceChunk : count;
+}
+
+// Hands a finished subtree root (position `pos` of its chunk's bitmap of roots
+// that go up) to the level above: staged in leaf order in the chunk's slots,
+// or, when the chunk has more roots than slots, climbed device-wide at once —
+// every root of such a chunk does, so that no node above mixes the LDS and the
+// device-wide arrival protocols.
+template <bool kKeys>
+__device__ void hand_up(const uint32_t* s_up, uint32_t words_up, uint32_t pos, uint32_t slot,
+                        palz_node_index node, uint32_t lo, uint32_t hi, v3 blo, v3 bhi, uint32_t n,
+                        const uint64_t* __restrict__ words, const uint8_t* __restrict__ deltas,
+                        uint64_t* __restrict__ arrival, palz_internal_node* nodes, palz_node_index* __restrict__ root,
+                        uint32_t ep, SubtreeRoot* __restrict__ out) {
+    if (staged_total(s_up, words_up) == kDeviceChunk) {
+        climb_device<kKeys>(node, lo, hi, blo, bhi, n, words, deltas, arrival, nodes, root, ep);
+        return;
+    }
+    const uint32_t word = pos >> 5;
+    uint32_t rank = __popc(s_up[word] & ((1u << (pos & 31u)) - 1u));
+    for (uint32_t w = 0; w < word; ++w) rank += __popc(s_up[w]);
+    SubtreeRoot r;
+    r.node = node;
+    r.lo = lo;
+    r.hi = hi;
+    r.pad_ = 0u;
+    r.box[0] = blo.x;
+    r.box[1] = blo.y;
+    r.box[2] = blo.z;
+    r.box[3] = bhi.x;
+    r.box[4] = bhi.y;
+    r.box[5] = bhi.z;
+    r.pad2_[0] = r.pad2_[1] = 0u;
+    const float4* src = reinterpret_cast<const float4*>(&r);
+    float4* dst = reinterpret_cast<float4*>(out + (size_t)slot * kChunkRoots + rank);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) dst[q] = src[q];
+}
+
+// computeDeltas + build (minimum_lbvh.cu:94-121, minimum_lbvh.h:293-370), one
+// workgroup per chunk of kBuildChunk sorted leaves.
+//
+// Node p (the split between leaves p and p+1) is "local" when both its
+// children lie in the chunk, i.e. its leaf range [lo, hi] does; that range is
+// fixed by the deltas alone (lo - 1 = nearest split on the left with delta >=
+// delta[p], hi = nearest split on the right with delta > delta[p]), so prefix
+// / suffix maxima of the chunk's deltas in LDS decide it before the climb.
+// Local nodes hand their first child over in LDS and their records are written
+// out whole at the end of the chunk; no local node needs device-wide
+// coherence.  A climb that reaches a non-local node stops: its subtree root is
+// staged (at most kChunkRoots per chunk, in leaf order) for k_build_level,
+// whose workgroups run the same climb over the roots of kGroupChunks chunks,
+// level by level, until one group holds them all.  A chunk or group with more
+// roots than slots sends them up the device-wide climb instead (and so does
+// every group above it).  For Morton-ordered input a chunk has a few roots;
+// long runs of equal keys (right-leaning chains) make more of them.  Both
+// children of a node always agree on how it is reached, and the boxes are
+// exact min/max, so the tree is the bit-identical result of the
+// one-thread-per-leaf climb.
+struct ChunkShared {
+    palz_internal_node node[kBuildChunk];  // records of the local nodes (read as float4)
+    uint8_t delta[kBuildChunk + 2];        // splits c0-1 .. c_last at [0 .. cnt+1]
+    uint8_t pre[kBuildChunk];              // max of the chunk's split deltas up to t
+    uint8_t suf[kBuildChunk];              // max of the chunk's split deltas from t on
+    uint8_t local[kBuildChunk];
+    uint32_t key[kBuildChunk + 2];         // keys c0-1 .. c_last+1 at [0 .. cnt+2]
+    uint32_t flag[kBuildChunk];            // arrival words of the local nodes
+    uint32_t up[kBuildChunk / 32];         // leaves whose subtree roots go up
+    uint32_t wave[2][kBuildChunk / 64];
 };
 
-template <bool kLeaves>
 __global__ void __launch_bounds__(kBuildChunk)
-    k_build(const palz_triangle* __restrict__ tris, const uint64_t* __restrict__ words,
-            const BuildItem* __restrict__ items, uint32_t m, uint32_t n, const uint8_t* __restrict__ deltas,
-            palz_internal_node* __restrict__ nodes, palz_node_index* __restrict__ root,
-            BuildItem* __restrict__ staging, uint32_t* __restrict__ chunk_count,
-            uint32_t* __restrict__ chunk_bits) {
-    __shared__ uint8_t s_delta[kBuildChunk];  // delta of the split after each of the chunk's items
-    __shared__ uint8_t s_pre[kBuildChunk];    // max of s_delta up to t
-    __shared__ uint8_t s_suf[kBuildChunk];    // max of s_delta from t on
-    __shared__ uint8_t s_local[kBuildChunk];
-    __shared__ uint32_t s_flag[kBuildChunk];
-    __shared__ uint32_t s_up[kBuildChunk / 32];             // chunk items that go up a level
-    __shared__ uint32_t s_wave[2][kBuildChunk / 64];        // per-wave prefix / suffix maxima
-    __shared__ palz_internal_node s_node[kBuildChunk];      // records of the local nodes
+    k_build(const palz_triangle* __restrict__ tris, const uint64_t* __restrict__ words, uint32_t n,
+            uint8_t* __restrict__ deltas, palz_internal_node* nodes, palz_node_index* __restrict__ root,
+            uint64_t* __restrict__ arrival, const BuildCtl* __restrict__ ctl,
+            SubtreeRoot* __restrict__ staged, uint32_t* __restrict__ staged_count) {
+    __shared__ ChunkShared s;
     const uint32_t n_int = n - 1u;
-    const uint32_t m_int = m - 1u;  // item-level splits
     const uint32_t c0 = blockIdx.x * kBuildChunk;
-    const uint32_t c_last = min(c0 + (uint32_t)kBuildChunk, m) - 1u;  // last item of the chunk
-    const uint32_t cnt = c_last - c0;                                   // item splits c0 .. c_last-1
+    const uint32_t c_last = min(c0 + (uint32_t)kBuildChunk, n) - 1u;  // last leaf of the chunk
+    const uint32_t cnt = c_last - c0;                                   // splits c0 .. c_last-1
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    auto item_hi = [&](uint32_t j) -> uint32_t { return kLeaves ? j : items[j].hi; };
-    auto split_delta = [&](uint32_t j) -> uint32_t { return (uint32_t)deltas[item_hi(j)]; };
-    const uint32_t d = t < cnt ? split_delta(c0 + t) : 0u;
-    s_delta[t] = (uint8_t)d;
-    s_flag[t] = kNoArrival;
-    if (t < kBuildChunk / 32) s_up[t] = 0u;
-    // prefix and suffix maxima: within the wave by shuffles, then across waves
+    const uint32_t it = c0 + t;
+    const bool active = it <= c_last;
+
+    uint32_t tri_index = 0u;
+    if (active) {
+        const uint64_t w = words[it];
+        tri_index = (uint32_t)(w >> 32);
+        s.key[t + 1u] = (uint32_t)w;
+    }
+    if (t == 0u && c0 > 0u) s.key[0] = key_at(words, c0 - 1u);
+    if (t == 1u && c_last < n_int) s.key[cnt + 2u] = key_at(words, c_last + 1u);
+    s.flag[t] = kNoArrival;
+    if (t < kBuildChunk / 32) s.up[t] = 0u;
+    __syncthreads();
+    // the delta of split c0 + t (t <= cnt) and of split c0 - 1
+    uint32_t d = 0u;
+    if (active && it < n_int) {
+        d = delta32(s.key[t + 1u], s.key[t + 2u]);
+        deltas[it] = (uint8_t)d;
+        s.delta[t + 1u] = (uint8_t)d;
+    }
+    if (t == 0u && c0 > 0u) s.delta[0] = (uint8_t)delta32(s.key[0], s.key[1]);
+    palz_triangle tr;
+    if (active) tr = tris[tri_index];
+    // prefix and suffix maxima of the split deltas c0 .. c_last-1: within the
+    // wave by shuffles, then across waves
+    if (t >= cnt) d = 0u;
     uint32_t pm = d, sm = d;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -295,145 +401,266 @@ __global__ void __launch_bounds__(kBuildChunk)
         if (lane >= (uint32_t)o) pm = max(pm, x);
         if (lane + o < 64u) sm = max(sm, y);
     }
-    if (lane == 63u) s_wave[0][wave] = pm;
-    if (lane == 0u) s_wave[1][wave] = sm;
+    if (lane == 63u) s.wave[0][wave] = pm;
+    if (lane == 0u) s.wave[1][wave] = sm;
     __syncthreads();
-    for (uint32_t w = 0; w < wave; ++w) pm = max(pm, s_wave[0][w]);
-    for (uint32_t w = wave + 1; w < kBuildChunk / 64; ++w) sm = max(sm, s_wave[1][w]);
-    s_pre[t] = (uint8_t)pm;
-    s_suf[t] = (uint8_t)sm;
+    for (uint32_t w = 0; w < wave; ++w) pm = max(pm, s.wave[0][w]);
+    for (uint32_t w = wave + 1; w < kBuildChunk / 64; ++w) sm = max(sm, s.wave[1][w]);
+    s.pre[t] = (uint8_t)pm;
+    s.suf[t] = (uint8_t)sm;
     __syncthreads();
     if (t < cnt) {
-        const uint32_t dp = s_delta[t];
-        const bool left = (t > 0u && s_pre[t - 1u] >= dp) || c0 == 0u || split_delta(c0 - 1u) >= dp;
-        const bool right = (t + 1u < cnt && s_suf[t + 1u] > dp) || c_last == m_int || split_delta(c_last) > dp;
-        s_local[t] = (left && right) ? 1 : 0;
+        const uint32_t dp = s.delta[t + 1u];
+        const bool left = (t > 0u && s.pre[t - 1u] >= dp) || c0 == 0u || s.delta[0] >= dp;
+        const bool right = (t + 1u < cnt && s.suf[t + 1u] > dp) || c_last == n_int || s.delta[cnt + 1u] > dp;
+        s.local[t] = (left && right) ? 1 : 0;
     }
     __syncthreads();
 
-    const uint32_t it = c0 + t;
-    if (it < m) {
-        uint32_t a = it, bb = it;  // item range
-        uint32_t lo, hi;           // leaf range
-        palz_node_index node;
-        v3 blo, bhi;
-        if (kLeaves) {
-            lo = hi = it;
-            const uint32_t tri_index = (uint32_t)(words[it] >> 32);
-            node = tri_index | PALZ_NODE_LEAF_BIT;
-            const palz_triangle tr = tris[tri_index];
-            blo = mk(tr.vs[0]);
-            bhi = mk(tr.vs[0]);
-            blo = cmin(blo, mk(tr.vs[1]));
-            bhi = cmax(bhi, mk(tr.vs[1]));
-            blo = cmin(blo, mk(tr.vs[2]));
-            bhi = cmax(bhi, mk(tr.vs[2]));
-        } else {
-            const BuildItem in = items[it];
-            lo = in.lo;
-            hi = in.hi;
-            node = in.node;
-            blo = mk(in.box[0], in.box[1], in.box[2]);
-            bhi = mk(in.box[3], in.box[4], in.box[5]);
-        }
-        auto delta_at = [&](uint32_t j) -> uint32_t {  // item split j
-            return j - c0 < cnt ? s_delta[j - c0] : split_delta(j);
-        };
-        bool go_up = false;
-        while (bb - a < m_int) {
-            const uint32_t dl = a > 0u ? delta_at(a - 1u) : 0xFFFFFFFFu;
-            const uint32_t dr = bb < m_int ? delta_at(bb) : 0xFFFFFFFFu;
+    uint32_t lo = it, hi = it;  // leaf range
+    palz_node_index node = tri_index | PALZ_NODE_LEAF_BIT;
+    v3 blo = mk(0.f, 0.f, 0.f), bhi = blo;
+    bool go_up = false;
+    if (active) {
+        blo = mk(tr.vs[0]);
+        bhi = mk(tr.vs[0]);
+        blo = cmin(blo, mk(tr.vs[1]));
+        bhi = cmax(bhi, mk(tr.vs[1]));
+        blo = cmin(blo, mk(tr.vs[2]));
+        bhi = cmax(bhi, mk(tr.vs[2]));
+        auto delta_at = [&](uint32_t j) -> uint32_t { return s.delta[j - c0 + 1u]; };  // j in [c0-1, c_last]
+        while (hi - lo < n_int) {
+            const uint32_t dl = lo > 0u ? delta_at(lo - 1u) : 0xFFFFFFFFu;
+            const uint32_t dr = hi < n_int ? delta_at(hi) : 0xFFFFFFFFu;
             const int go_left = dl < dr ? 1 : 0;
-            const uint32_t parent = go_left ? a - 1u : bb;  // item split
-            if (!(parent - c0 < cnt && s_local[parent - c0])) {
+            const uint32_t parent = go_left ? lo - 1u : hi;
+            if (!(parent - c0 < cnt && s.local[parent - c0])) {
                 go_up = true;
                 break;
             }
-            palz_internal_node& rec = s_node[parent - c0];
+            palz_internal_node& rec = s.node[parent - c0];
             rec.children[go_left] = node;
             rec.aabbs[go_left].lower = {blo.x, blo.y, blo.z};
             rec.aabbs[go_left].upper = {bhi.x, bhi.y, bhi.z};
+            const uint32_t mine = go_left ? hi : lo;
+            const uint32_t other = __hip_atomic_exchange(&s.flag[parent - c0], mine, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (other == kNoArrival) break;
+            const palz_aabb sb = rec.aabbs[go_left ^ 1];
+            lo = min(lo, other);
+            hi = max(hi, other);
+            uint32_t grand = PALZ_NODE_INVALID;
+            if (hi - lo < n_int) {
+                const uint32_t gl = lo > 0u ? delta_at(lo - 1u) : 0xFFFFFFFFu;
+                const uint32_t gr = hi < n_int ? delta_at(hi) : 0xFFFFFFFFu;
+                grand = gl < gr ? lo - 1u : hi;
+            }
+            rec.parent = grand;
+            rec.context = hi < n_int ? hi : 0xFFFFFFFFu;
+            node = parent;
+            blo = cmin(blo, mk(sb.lower));
+            bhi = cmax(bhi, mk(sb.upper));
+        }
+        if (go_up)
+            atomicOr(&s.up[t >> 5], 1u << (t & 31u));
+        else if (hi - lo == n_int)
+            *root = node;
+    }
+    __syncthreads();
+    if (go_up) hand_up<true>(s.up, kBuildChunk / 32, t, blockIdx.x, node, lo, hi, blo, bhi, n, words, nullptr,
+                             arrival, nodes, root, ctl->epoch, staged);
+    if (t == 0u && staged_count) staged_count[blockIdx.x] = staged_total(s.up, kBuildChunk / 32);
+    // the chunk's local records, written out whole
+    for (uint32_t i = t; i < cnt; i += blockDim.x) {
+        if (!s.local[i]) continue;
+        const float4* src = reinterpret_cast<const float4*>(&s.node[i]);
+        float4* dst = reinterpret_cast<float4*>(nodes + c0 + i);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dst[q] = src[q];
+    }
+}
+
+// The levels above the first: workgroup g takes the staged subtree roots of
+// lower chunks [16g, 16g + 16) (at most kGroupItems, in leaf order) and runs the
+// same LDS climb on them, with the split after each item at the item's last
+// leaf.  A node is local to the group when its leaf range lies within the
+// group's items.  A group with a lower chunk whose roots climbed device-wide
+// sends all of its items device-wide too (the nodes above them have a
+// device-wide child).  The last level has one group, which reaches the root.
+struct GroupShared {
+    palz_internal_node node[kGroupItems];  // records of the local nodes (read as float4)
+    uint8_t delta[kGroupItems + 1];        // split before item 0, after items 0 .. M-1
+    uint8_t pre[kGroupItems];
+    uint8_t suf[kGroupItems];
+    uint8_t local[kGroupItems];
+    uint32_t flag[kGroupItems];
+    uint32_t leaf_hi[kGroupItems];         // leaf-level split after each item
+    uint32_t up[kGroupItems / 32];
+    uint32_t wave[2][kGroupItems / 64];
+    uint32_t offset[kGroupChunks + 1];
+    uint32_t any_device, first_lo, last_hi;
+};
+
+__global__ void __launch_bounds__(kGroupItems)
+    k_build_level(const SubtreeRoot* __restrict__ in, const uint32_t* __restrict__ in_count, uint32_t in_chunks,
+                  uint32_t n, const uint8_t* __restrict__ deltas, palz_internal_node* nodes,
+                  palz_node_index* __restrict__ root, uint64_t* __restrict__ arrival,
+                  const BuildCtl* __restrict__ ctl, SubtreeRoot* __restrict__ out, uint32_t* __restrict__ out_count) {
+    __shared__ GroupShared s;
+    const uint32_t n_int = n - 1u;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const uint32_t g = blockIdx.x;
+    const uint32_t k0 = g * kGroupChunks;
+    const uint32_t nlow = min((uint32_t)kGroupChunks, in_chunks - k0);
+    const uint32_t ep = ctl->epoch;
+    if (t == 0u) {
+        uint32_t acc = 0u, dev = 0u;
+        for (uint32_t k = 0; k < nlow; ++k) {
+            const uint32_t c = in_count[k0 + k];
+            s.offset[k] = acc;
+            if (c == kDeviceChunk)
+                dev = 1u;
+            else
+                acc += c;
+        }
+        s.offset[nlow] = acc;
+        s.any_device = dev;
+    }
+    s.flag[t] = kNoArrival;
+    if (t < kGroupItems / 32) s.up[t] = 0u;
+    __syncthreads();
+    if (s.any_device) {
+        const uint32_t k = t / (uint32_t)kChunkRoots, r = t % (uint32_t)kChunkRoots;
+        if (k < nlow) {
+            const uint32_t c = in_count[k0 + k];
+            if (c != kDeviceChunk && r < c) {
+                const SubtreeRoot x = in[(size_t)(k0 + k) * kChunkRoots + r];
+                climb_device<false>(x.node, x.lo, x.hi, mk(x.box[0], x.box[1], x.box[2]),
+                                    mk(x.box[3], x.box[4], x.box[5]), n, nullptr, deltas, arrival, nodes, root, ep);
+            }
+        }
+        if (t == 0u && out_count) out_count[g] = kDeviceChunk;
+        return;
+    }
+    const uint32_t M = s.offset[nlow];  // >= nlow: every lower chunk has a root
+    const bool active = t < M;
+    const uint32_t cnt = M - 1u;        // item splits 0 .. M-2
+    uint32_t lo = 0u, hi = 0u;
+    palz_node_index node = 0u;
+    v3 blo = mk(0.f, 0.f, 0.f), bhi = blo;
+    uint32_t d = 0u;
+    if (active) {
+        uint32_t k = 0u;
+        while (s.offset[k + 1u] <= t) ++k;
+        const SubtreeRoot x = in[(size_t)(k0 + k) * kChunkRoots + (t - s.offset[k])];
+        node = x.node;
+        lo = x.lo;
+        hi = x.hi;
+        blo = mk(x.box[0], x.box[1], x.box[2]);
+        bhi = mk(x.box[3], x.box[4], x.box[5]);
+        s.leaf_hi[t] = hi;
+        if (hi < n_int) {
+            d = deltas[hi];
+            s.delta[t + 1u] = (uint8_t)d;
+        }
+        if (t == 0u) {
+            s.first_lo = lo;
+            if (lo > 0u) s.delta[0] = deltas[lo - 1u];
+        }
+        if (t == cnt) s.last_hi = hi;
+    }
+    // prefix and suffix maxima of the item split deltas 0 .. M-2
+    if (t >= cnt) d = 0u;
+    uint32_t pm = d, sm = d;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t x = (uint32_t)__shfl_up((int)pm, o, 64);
+        const uint32_t y = (uint32_t)__shfl_down((int)sm, o, 64);
+        if (lane >= (uint32_t)o) pm = max(pm, x);
+        if (lane + o < 64u) sm = max(sm, y);
+    }
+    if (lane == 63u) s.wave[0][wave] = pm;
+    if (lane == 0u) s.wave[1][wave] = sm;
+    __syncthreads();
+    for (uint32_t w = 0; w < wave; ++w) pm = max(pm, s.wave[0][w]);
+    for (uint32_t w = wave + 1; w < kGroupItems / 64; ++w) sm = max(sm, s.wave[1][w]);
+    s.pre[t] = (uint8_t)pm;
+    s.suf[t] = (uint8_t)sm;
+    __syncthreads();
+    if (t < cnt) {
+        const uint32_t dp = s.delta[t + 1u];
+        const bool left = (t > 0u && s.pre[t - 1u] >= dp) || s.first_lo == 0u || s.delta[0] >= dp;
+        const bool right = (t + 1u < cnt && s.suf[t + 1u] > dp) || s.last_hi == n_int || s.delta[M] > dp;
+        s.local[t] = (left && right) ? 1 : 0;
+    }
+    __syncthreads();
+
+    bool go_up = false;
+    if (active) {
+        uint32_t a = t, bb = t;  // item range, in the group's numbering
+        while (hi - lo < n_int) {
+            const uint32_t dl = lo > 0u ? s.delta[a] : 0xFFFFFFFFu;         // split a-1
+            const uint32_t dr = hi < n_int ? s.delta[bb + 1u] : 0xFFFFFFFFu;  // split bb
+            const int go_left = dl < dr ? 1 : 0;
+            const uint32_t parent = go_left ? a - 1u : bb;  // item split (a - 1 wraps for a = 0)
+            if (!(parent < cnt && s.local[parent])) {
+                go_up = true;
+                break;
+            }
+            palz_internal_node& rec = s.node[parent];
+            rec.children[go_left] = node;
+            rec.aabbs[go_left].lower = {blo.x, blo.y, blo.z};
+            rec.aabbs[go_left].upper = {bhi.x, bhi.y, bhi.z};
+            // The far leaf end travels in the field the other child leaves alone
+            // (the second arriver overwrites both below).
+            if (go_left)
+                rec.context = hi;
+            else
+                rec.parent = lo;
+            const uint32_t split_leaf = go_left ? lo - 1u : hi;
             const uint32_t mine = go_left ? bb : a;
-            const uint32_t other = __hip_atomic_exchange(&s_flag[parent - c0], mine, __ATOMIC_ACQ_REL,
+            const uint32_t other = __hip_atomic_exchange(&s.flag[parent], mine, __ATOMIC_ACQ_REL,
                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
             if (other == kNoArrival) break;
             const palz_aabb sb = rec.aabbs[go_left ^ 1];
             if (go_left) {
                 a = other;
-                lo = kLeaves ? other : items[other].lo;
+                lo = rec.parent;
             } else {
                 bb = other;
-                hi = kLeaves ? other : items[other].hi;
+                hi = rec.context;
             }
             uint32_t grand = PALZ_NODE_INVALID;
             if (hi - lo < n_int) {
-                const uint32_t gl = lo > 0u ? (uint32_t)deltas[lo - 1u] : 0xFFFFFFFFu;
-                const uint32_t gr = hi < n_int ? (uint32_t)deltas[hi] : 0xFFFFFFFFu;
+                const uint32_t gl = lo > 0u ? s.delta[a] : 0xFFFFFFFFu;
+                const uint32_t gr = hi < n_int ? s.delta[bb + 1u] : 0xFFFFFFFFu;
                 grand = gl < gr ? lo - 1u : hi;
             }
             rec.parent = grand;
             rec.context = hi < n_int ? hi : 0xFFFFFFFFu;
-            node = kLeaves ? parent : item_hi(parent);  // the split's leaf-level index
+            node = split_leaf;
             blo = cmin(blo, mk(sb.lower));
             bhi = cmax(bhi, mk(sb.upper));
         }
-        if (bb - a == m_int) {
+        if (go_up)
+            atomicOr(&s.up[t >> 5], 1u << (t & 31u));
+        else if (hi - lo == n_int)
             *root = node;
-        } else if (go_up) {
-            // this subtree root is an item of the next level, staged at its first item
-            BuildItem up;
-            up.node = node;
-            up.lo = lo;
-            up.hi = hi;
-            up.box[0] = blo.x;
-            up.box[1] = blo.y;
-            up.box[2] = blo.z;
-            up.box[3] = bhi.x;
-            up.box[4] = bhi.y;
-            up.box[5] = bhi.z;
-            up.pad_ = 0u;
-            staging[a] = up;
-            atomicOr(&s_up[(a - c0) >> 5], 1u << ((a - c0) & 31u));
-        }
     }
     __syncthreads();
-    // the chunk's local records, written out whole
+    if (out_count) {
+        if (go_up)
+            hand_up<false>(s.up, kGroupItems / 32, t, g, node, lo, hi, blo, bhi, n, nullptr, deltas, arrival, nodes,
+                           root, ep, out);
+        if (t == 0u) out_count[g] = staged_total(s.up, kGroupItems / 32);
+    }
+    // the group's local records, written out whole
     for (uint32_t i = t; i < cnt; i += blockDim.x) {
-        if (!s_local[i]) continue;
-        const float4* src = reinterpret_cast<const float4*>(&s_node[i]);
-        float4* dst = reinterpret_cast<float4*>(nodes + item_hi(c0 + i));
+        if (!s.local[i]) continue;
+        const float4* src = reinterpret_cast<const float4*>(&s.node[i]);
+        float4* dst = reinterpret_cast<float4*>(nodes + s.leaf_hi[i]);
 #pragma unroll
         for (int q = 0; q < 4; ++q) dst[q] = src[q];
-    }
-    // which of the chunk's positions hold an item of the next level
-    if (staging) {
-        if (t < kBuildChunk / 32) chunk_bits[blockIdx.x * (kBuildChunk / 32) + t] = s_up[t];
-        if (t == 0) {
-            uint32_t total = 0;
-            for (uint32_t w = 0; w < kBuildChunk / 32; ++w) total += __popc(s_up[w]);
-            chunk_count[blockIdx.x] = total;
-        }
-    }
-}
-
-// Compacts the staged items of each chunk (staging[c * kBuildChunk + j] for
-// the set bits j of its bitmap) to out[offset[c] + rank]: the next level's
-// items in leaf order.  One workgroup of kBuildChunk threads per chunk.
-__global__ void __launch_bounds__(kBuildChunk) k_build_gather(const BuildItem* __restrict__ staging,
-                                                              const uint32_t* __restrict__ bits,
-                                                              const uint32_t* __restrict__ offset,
-                                                              uint32_t n_chunks, BuildItem* __restrict__ out) {
-    __shared__ uint32_t s_bits[kBuildChunk / 32];
-    for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
-        __syncthreads();
-        if (threadIdx.x < kBuildChunk / 32) s_bits[threadIdx.x] = bits[c * (kBuildChunk / 32) + threadIdx.x];
-        __syncthreads();
-        const uint32_t t = threadIdx.x, word = t >> 5, bit = 1u << (t & 31u);
-        if (s_bits[word] & bit) {
-            uint32_t rank = __popc(s_bits[word] & (bit - 1u));
-            for (uint32_t w = 0; w < word; ++w) rank += __popc(s_bits[w]);
-            out[offset[c] + rank] = staging[(size_t)c * kBuildChunk + t];
-        }
     }
 }
 
@@ -448,16 +675,14 @@ struct palz_lbvh {
     uint64_t* words = nullptr;
     uint64_t* tmp = nullptr;
     uint8_t* deltas = nullptr;
-    uint32_t* arrival = nullptr;
-    palz::BuildItem* staging = nullptr;  // per chunk, the items that go up a level
-    palz::BuildItem* items[2] = {};      // the items of a level (ping-pong)
-    uint32_t items_cap = 0;
-    uint32_t* chunk_count = nullptr;     // [chunks] counts, [chunks] offsets, scan sums, total
-    uint32_t* chunk_bits = nullptr;      // per chunk, the positions staged for the next level
+    uint64_t* arrival = nullptr;  // per split: (epoch, far leaf end) of the device-wide climb
+    palz::SubtreeRoot* staged[2] = {};   // per chunk / group, the subtree roots that go up (ping-pong)
+    uint32_t* staged_count[2] = {};
+    palz::BuildCtl* ctl = nullptr;
     uint32_t capacity = 0;
     int n = 0;
     // Optional stage timing: events after each stage of the last build.
-    static constexpr int kStages = 8;
+    static constexpr int kStages = 7;
     bool timing = false;
     hipEvent_t ev[kStages + 1] = {};
 
@@ -468,30 +693,24 @@ struct palz_lbvh {
         PALZ_HIP(hipMalloc(&words, sizeof(uint64_t) * want));
         PALZ_HIP(hipMalloc(&tmp, sizeof(uint64_t) * want));
         PALZ_HIP(hipMalloc(&deltas, std::max(4u, want)));
-        PALZ_HIP(hipMalloc(&arrival, sizeof(uint32_t) * std::max(4u, want)));
+        // arrival words start at epoch 0, which no build uses
+        PALZ_HIP(hipMalloc(&arrival, sizeof(uint64_t) * std::max(1u, want)));
+        PALZ_HIP(hipMemset(arrival, 0, sizeof(uint64_t) * std::max(1u, want)));
         const uint32_t chunks = palz::div_up(want, (uint32_t)palz::kBuildChunk);
-        PALZ_HIP(hipMalloc(&staging, sizeof(palz::BuildItem) * (size_t)chunks * palz::kBuildChunk));
-        PALZ_HIP(hipMalloc(&chunk_count, sizeof(uint32_t) * (2u * chunks + palz::scan_scratch_entries(chunks) + 4u)));
-        PALZ_HIP(hipMalloc(&chunk_bits, sizeof(uint32_t) * chunks * (palz::kBuildChunk / 32)));
+        const uint32_t groups = palz::div_up(chunks, (uint32_t)palz::kGroupChunks);
+        PALZ_HIP(hipMalloc(&staged[0], sizeof(palz::SubtreeRoot) * palz::kChunkRoots * chunks));
+        PALZ_HIP(hipMalloc(&staged[1], sizeof(palz::SubtreeRoot) * palz::kChunkRoots * groups));
+        PALZ_HIP(hipMalloc(&staged_count[0], sizeof(uint32_t) * chunks));
+        PALZ_HIP(hipMalloc(&staged_count[1], sizeof(uint32_t) * groups));
         capacity = want;
         return PALZ_OK;
     }
     void release_arrays() {
-        if (nodes) (void)hipFree(nodes);
-        if (words) (void)hipFree(words);
-        if (tmp) (void)hipFree(tmp);
-        if (deltas) (void)hipFree(deltas);
-        if (arrival) (void)hipFree(arrival);
-        if (staging) (void)hipFree(staging);
-        if (chunk_count) (void)hipFree(chunk_count);
-        if (chunk_bits) (void)hipFree(chunk_bits);
-        chunk_bits = nullptr;
-        for (auto& p : items)
+        for (void* p : {(void*)nodes, (void*)words, (void*)tmp, (void*)deltas, (void*)arrival, (void*)staged[0],
+                        (void*)staged[1], (void*)staged_count[0], (void*)staged_count[1]})
             if (p) (void)hipFree(p);
-        staging = nullptr;
-        chunk_count = nullptr;
-        items[0] = items[1] = nullptr;
-        items_cap = 0;
+        staged[0] = staged[1] = nullptr;
+        staged_count[0] = staged_count[1] = nullptr;
         nodes = nullptr;
         words = tmp = nullptr;
         deltas = nullptr;
@@ -504,6 +723,7 @@ struct palz_lbvh {
         release_arrays();
         if (root) (void)hipFree(root);
         if (scene) (void)hipFree(scene);
+        if (ctl) (void)hipFree(ctl);
     }
 };
 
@@ -512,11 +732,13 @@ extern "C" {
 int palz_lbvh_create(palz_lbvh** out) {
     if (!out) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_lbvh_create: out is NULL");
     palz_lbvh* b = new palz_lbvh();
-    hipError_t e1 = hipMalloc(&b->root, sizeof(palz_node_index));
-    hipError_t e2 = hipMalloc(&b->scene, sizeof(palz_aabb));
-    if (e1 != hipSuccess || e2 != hipSuccess) {
+    hipError_t e = hipMalloc(&b->root, sizeof(palz_node_index));
+    if (e == hipSuccess) e = hipMalloc(&b->scene, sizeof(palz_aabb));
+    if (e == hipSuccess) e = hipMalloc(&b->ctl, sizeof(palz::BuildCtl));
+    if (e == hipSuccess) e = hipMemset(b->ctl, 0, sizeof(palz::BuildCtl));
+    if (e != hipSuccess) {
         delete b;
-        return palz::hip_fail(e1 != hipSuccess ? e1 : e2, "hipMalloc", __FILE__, __LINE__);
+        return palz::hip_fail(e, "hipMalloc", __FILE__, __LINE__);
     }
     *out = b;
     return PALZ_OK;
@@ -562,57 +784,25 @@ int palz_lbvh_build(palz_lbvh* b, const palz_triangle* d_tris, int n, uint32_t b
     }
     if (ev) PALZ_HIP(hipEventRecord(ev[1], st));
     k_morton_hist<<<std::min(div_up(un, (uint32_t)kTriTile), 2048u), kTriTile, 0, st>>>(
-        d_tris, un, aligned, b->scene, build_option, morton_bits, b->words, s.hist);
+        d_tris, un, aligned, b->scene, build_option, morton_bits, b->words, s.hist, b->ctl);
     if (ev) PALZ_HIP(hipEventRecord(ev[2], st));
     const int npass = launch_sort_passes(s, b->words, b->tmp, nullptr, nullptr, un, 0, 32, st,
                                          ev ? ev + 3 : nullptr);
     (void)npass;  // 4 passes: the sorted words are back in b->words
-    if (un > 1u) {
-        const uint32_t quads = div_up(un - 1u, 4u);
-        k_deltas<<<div_up(quads, 256u), 256, 0, st>>>(b->words, un, b->deltas, b->arrival);
+    // level 1 over chunks of leaves, then one launch per level of groups of
+    // kGroupChunks lower chunks, until one group holds everything
+    uint32_t chunks = div_up(un, (uint32_t)kBuildChunk);
+    k_build<<<chunks, kBuildChunk, 0, st>>>(d_tris, b->words, un, b->deltas, b->nodes, b->root, b->arrival, b->ctl,
+                                            b->staged[0], chunks > 1u ? b->staged_count[0] : nullptr);
+    for (int cur = 0; chunks > 1u; cur ^= 1) {
+        const uint32_t groups = div_up(chunks, (uint32_t)kGroupChunks);
+        k_build_level<<<groups, kGroupItems, 0, st>>>(b->staged[cur], b->staged_count[cur], chunks, un, b->deltas,
+                                                      b->nodes, b->root, b->arrival, b->ctl, b->staged[cur ^ 1],
+                                                      groups > 1u ? b->staged_count[cur ^ 1] : nullptr);
+        chunks = groups;
     }
+    PALZ_LAUNCH_CHECK();
     if (ev) PALZ_HIP(hipEventRecord(ev[7], st));
-    // level by level, until one chunk holds every item
-    {
-        uint32_t m = un;
-        int cur = -1;  // level 1 reads the leaves
-        for (int level = 0;; ++level) {
-            const uint32_t chunks = div_up(m, (uint32_t)kBuildChunk);
-            BuildItem* stage = chunks > 1u ? b->staging : nullptr;
-            if (cur < 0)
-                k_build<true><<<chunks, kBuildChunk, 0, st>>>(d_tris, b->words, nullptr, m, un, b->deltas, b->nodes,
-                                                              b->root, stage, b->chunk_count, b->chunk_bits);
-            else
-                k_build<false><<<chunks, kBuildChunk, 0, st>>>(nullptr, nullptr, b->items[cur], m, un, b->deltas,
-                                                               b->nodes, b->root, stage, b->chunk_count,
-                                                               b->chunk_bits);
-            PALZ_LAUNCH_CHECK();
-            if (chunks == 1u) break;
-            if (level > 64) return fail(PALZ_ERR_HIP, "palz_lbvh_build: the level-wise build does not converge");
-            uint32_t* counts = b->chunk_count;
-            uint32_t* offsets = counts + chunks;
-            uint32_t* sums = offsets + chunks;
-            uint32_t* total = sums + scan_scratch_entries(chunks);
-            launch_exclusive_scan(counts, offsets, chunks, nullptr, total, sums, st);
-            uint32_t next = 0;
-            PALZ_HIP(hipMemcpyAsync(&next, total, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-            PALZ_HIP(hipStreamSynchronize(st));
-            if (next > b->items_cap) {
-                for (auto& p : b->items)
-                    if (p) PALZ_HIP(hipFree(p));
-                const uint32_t cap = std::max(next, b->items_cap * 2u);
-                PALZ_HIP(hipMalloc(&b->items[0], sizeof(BuildItem) * cap));
-                PALZ_HIP(hipMalloc(&b->items[1], sizeof(BuildItem) * cap));
-                b->items_cap = cap;
-            }
-            const int out = cur < 0 ? 0 : cur ^ 1;
-            k_build_gather<<<std::min(chunks, 8192u), kBuildChunk, 0, st>>>(b->staging, b->chunk_bits, offsets,
-                                                                             chunks, b->items[out]);
-            cur = out;
-            m = next;
-        }
-    }
-    if (ev) PALZ_HIP(hipEventRecord(ev[8], st));
     PALZ_LAUNCH_CHECK();
     return PALZ_OK;
 }

@@ -93,6 +93,24 @@ def test_lbvh_bit_exact(sorter, n, bits):
     assert g_nodes.tobytes() == nodes.tobytes()
 
 
+@pytest.mark.parametrize("n,run", [(5000, 5000), (100_003, 20_000), (300_000, 3000), (40_000, 600)])
+def test_lbvh_long_equal_key_runs(sorter, n, run):
+    # a run of identical triangles gives equal Morton keys, whose splits all
+    # have delta 0: a right-leaning chain that no chunk of leaves can close
+    print(f"lbvh equal-key run n={n} run={run}", flush=True)
+    tris = scenes.triangle_soup(n, seed=n + run)
+    start = (n - run) // 3
+    tris[start:start + run] = tris[start]
+    for bits in (30, 32):
+        nodes, root, words, deltas, box = oracle.lbvh(tris, bits=bits)
+        assert np.count_nonzero(deltas == 0) >= run - 1
+        b = _gpu_lbvh(sorter, tris, bits)
+        g_words, g_deltas = b.download_sorted()
+        assert np.array_equal(g_deltas, deltas)
+        assert b.download_root() == root
+        assert b.download_nodes().tobytes() == nodes.tobytes()
+
+
 def test_lbvh_use_normal_option(sorter):
     tris = scenes.triangle_soup(5000, seed=77)
     nodes, root, words, deltas, box = oracle.lbvh(tris, option=T.BUILD_OPTION_USE_NORMAL, bits=32)
