@@ -39,12 +39,15 @@ constexpr float kCostTolerance = 1.0e-10f;  // main_gpu.cu:456
 constexpr int kWarmUp = 1;                  // main_gpu.cu:456
 constexpr int kQueueStride = 6;             // floats per converged solution (K <= 3)
 
+constexpr uint32_t kXcds = 8;  // MI355X: workgroups go round-robin over 8 XCDs, each with its own L2
+
 struct Counters {
     uint32_t ticket[3];
     uint32_t n_items;    // items of every chain length (scan length)
     uint32_t converged;  // scan total
     uint32_t pad_;
     unsigned long long stats[16];
+    uint32_t occlude[3][kXcds];  // per chain length and XCD: list positions handed out
 };
 
 __global__ void k_reset(Counters* c, const uint32_t* __restrict__ ranges, uint32_t L) {
@@ -286,8 +289,97 @@ __device__ __forceinline__ void refine_step(const float* prm, v3 pb, v3 pe, cons
     for (int i = 0; i < P; ++i) p64[i] = p64[i] - dp[i];
 }
 
-// Throughput, dA/dw and irradiance of the converged chains of length K, in
-// item order.  kRefined: the FP64 step runs here and its chain feeds dA/dw
+// contributableThroughput (pk.h:760-842) of the converged chains of length K,
+// in item order: every test but the occlusion rays per lane, then the K + 1
+// occlusion rays as wave packets (per lane when the wave's items belong to
+// more than one scene).  Writes the throughput (0 when any test fails) per
+// list position.  A kernel of its own so that the latency-bound traversal runs
+// at the occupancy of its few live registers, not at that of dA/dw's.
+//
+// The list keeps the tuple-coherent path order, so neighbouring positions
+// trace neighbouring rays.  Each XCD (the workgroups with blockIdx % 8 equal)
+// takes a contiguous eighth of the positions, so that its L2 holds the nodes
+// and triangles of one part of the scene rather than of all of it; its waves
+// take kOccludeGrab positions at a time from that eighth's counter, then help
+// the other XCDs' eighths.
+constexpr uint32_t kOccludeGrab = 256;
+
+// Clears chain length K's per-XCD counters before its k_occlude (a kernel, not
+// a memset, so that it can be captured in a graph).
+__global__ void k_clear_occlude(Counters* c, int K) {
+    if (threadIdx.x < kXcds) c->occlude[K - 1][threadIdx.x] = 0u;
+}
+
+template <int K>
+__global__ void __launch_bounds__(256)
+    k_occlude(const palz_scene_desc* __restrict__ scenes, const palz_path* __restrict__ paths,
+              const uint32_t* __restrict__ ranges, uint32_t L, Counters* __restrict__ ctr,
+              const uint32_t* __restrict__ excl, const uint32_t* __restrict__ list,
+              const float* __restrict__ conv_params, float* __restrict__ thr_out) {
+    constexpr int P = 2 * K;
+    const uint32_t n_items = ctr->n_items, total = ctr->converged;
+    const uint32_t begin = compact_pos(excl, ranges[K - 1] * L, n_items, total);
+    const uint32_t end = compact_pos(excl, ranges[K] * L, n_items, total);
+    __shared__ PacketStack packet_stacks[256 / 64];
+    PacketStack* pstack = &packet_stacks[threadIdx.x >> 6];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t per_xcd = div_up(end - begin, kXcds);
+    const uint32_t home = blockIdx.x % kXcds;
+    for (uint32_t k = 0; k < kXcds; ++k) {
+        const uint32_t x = (home + k) % kXcds;
+        const uint32_t r0 = begin + min(x * per_xcd, end - begin);
+        const uint32_t r1 = begin + min((x + 1u) * per_xcd, end - begin);
+        for (;;) {
+            uint32_t grab = 0u;
+            if (lane == 0u) grab = atomicAdd(&ctr->occlude[K - 1][x], kOccludeGrab);
+            grab = __builtin_amdgcn_readfirstlane(grab);
+            if (r0 + grab >= r1 || grab >= r1 - r0) break;
+            for (uint32_t j = 0; j < kOccludeGrab; j += 64u) {
+                const uint32_t s = r0 + grab + j + lane;
+                if (__ballot(s < r1) == 0ull) break;
+                const bool in_range = s < r1;
+                const uint32_t item = list[in_range ? s : r0];
+                const uint32_t pi = item / L, l = item % L;
+                const palz_path path = paths[pi];
+                const palz_scene_desc* sc = scenes + path.scene;
+                Segment seg[K + 1] = {};
+                float thr = 0.0f;
+                if (in_range) {
+                    ChainTri tri[K];
+#pragma unroll
+                    for (int t = 0; t < K; ++t) tri[t] = load_chain_tri(sc->tris, sc->attribs, path.tris[t]);
+                    float prm[P];
+#pragma unroll
+                    for (int i = 0; i < P; ++i) prm[i] = conv_params[(size_t)item * kQueueStride + i];
+                    const v3 pe = mk(sc->points[path.pixel].p);
+                    const v3 pb = mk(sc->p_light);
+                    thr = throughput_unoccluded<K>(prm, pb, pe, tri, path.event, item_eta(sc, l), seg);
+                }
+                const uint32_t scene0 = __builtin_amdgcn_readfirstlane(path.scene);
+                const bool one_scene = __ballot(path.scene != scene0) == 0ull;
+                bool visible = 0.0f < thr;
+#pragma unroll
+                for (int i = 0; i < K + 1; ++i) {
+                    const v3 ird = inv_rd(seg[i].rd);
+                    bool hit;
+                    if (one_scene) {
+                        const palz_scene_desc* us = scenes + scene0;
+                        hit = any_hit_packet(us->nodes, us->tris, *us->root, visible, seg[i].ro, seg[i].rd, ird,
+                                             1.0f, pstack);
+                    } else {
+                        hit = visible &&
+                              any_hit_escape(sc->nodes, sc->tris, *sc->root, seg[i].ro, seg[i].rd, ird, 1.0f);
+                    }
+                    if (hit) visible = false;
+                }
+                if (in_range) thr_out[s] = visible ? thr : 0.0f;
+            }
+        }
+    }
+}
+
+// dA/dw and irradiance of the converged chains of length K, in item order,
+// from the throughput k_occlude left per list position.  kRefined: the FP64 step runs here and its chain feeds dA/dw
 // (PALZ_REFINE_ALL); otherwise dA/dw uses the FP32 chain.
 template <int K, bool kRefined>
 __global__ void __launch_bounds__(256)
@@ -295,13 +387,12 @@ __global__ void __launch_bounds__(256)
              const uint32_t* __restrict__ ranges, uint32_t L, Counters* __restrict__ ctr,
              const uint32_t* __restrict__ excl, const uint32_t* __restrict__ list,
              const float* __restrict__ conv_params, const int* __restrict__ conv_iters,
-             palz_path_result* __restrict__ results, double* __restrict__ refined) {
+             const float* __restrict__ thr_in, palz_path_result* __restrict__ results,
+             double* __restrict__ refined) {
     constexpr int P = 2 * K;
     const uint32_t n_items = ctr->n_items, total = ctr->converged;
     const uint32_t begin = compact_pos(excl, ranges[K - 1] * L, n_items, total);
     const uint32_t end = compact_pos(excl, ranges[K] * L, n_items, total);
-    __shared__ PacketStack packet_stacks[256 / 64];
-    PacketStack* pstack = &packet_stacks[threadIdx.x >> 6];
     unsigned long long n_contrib = 0ull;
     for (uint32_t s = begin + blockIdx.x * blockDim.x + threadIdx.x; s < end; s += gridDim.x * blockDim.x) {
         const uint32_t item = list[s];
@@ -319,28 +410,7 @@ __global__ void __launch_bounds__(256)
         const v3 pb = mk(sc->p_light);
         const float eta = item_eta(sc, l);
         const uint32_t event = path.event;  // per lane (see k_solve_plain)
-        // contributableThroughput: every test but the occlusion rays per lane,
-        // then the K + 1 occlusion rays as wave packets (per lane when the
-        // wave's items belong to more than one scene)
-        Segment seg[K + 1] = {};
-        float thr = throughput_unoccluded<K>(prm, pb, pe, tri, event, eta, seg);
-        const uint32_t scene0 = __builtin_amdgcn_readfirstlane(path.scene);
-        const bool one_scene = __ballot(path.scene != scene0) == 0ull;
-        bool visible = 0.0f < thr;
-#pragma unroll
-        for (int i = 0; i < K + 1; ++i) {
-            const v3 ird = inv_rd(seg[i].rd);
-            bool hit;
-            if (one_scene) {
-                const palz_scene_desc* us = scenes + scene0;
-                hit = any_hit_packet(us->nodes, us->tris, *us->root, visible, seg[i].ro, seg[i].rd, ird, 1.0f,
-                                     pstack);
-            } else {
-                hit = visible && any_hit_escape(sc->nodes, sc->tris, *sc->root, seg[i].ro, seg[i].rd, ird, 1.0f);
-            }
-            if (hit) visible = false;
-        }
-        if (!visible) thr = 0.0f;
+        const float thr = thr_in[s];        // contributableThroughput, from k_occlude
         float pf[P];
 #pragma unroll
         for (int i = 0; i < P; ++i) pf[i] = prm[i];
@@ -455,15 +525,18 @@ void launch_newton(const palz_launch_config& c, int cus, hipStream_t st, const p
 template <int K>
 void launch_finish(const palz_launch_config& c, int cus, hipStream_t st, const palz_scene_desc* scenes,
                    const palz_path* paths, const uint32_t* ranges, uint32_t L, Counters* ctr, const uint32_t* excl,
-                   const uint32_t* list, const float* cp, const int* cit, int refine, palz_path_result* results,
-                   double* refined) {
+                   const uint32_t* list, const float* cp, const int* cit, float* thr, int refine,
+                   palz_path_result* results, double* refined) {
     const dim3 grid(cus * 8), block(c.finish_block);
+    k_clear_occlude<<<1, 64, 0, st>>>(ctr, K);
+    k_occlude<K><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, thr);
     if (refine == PALZ_REFINE_ALL) {
-        k_finish<K, true><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, cit, results,
+        k_finish<K, true><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, cit, thr, results,
                                                   refined);
         return;
     }
-    k_finish<K, false><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, cit, results, nullptr);
+    k_finish<K, false><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, cit, thr, results,
+                                               nullptr);
 }
 template <int K>
 void launch_refine(const palz_launch_config& c, int cus, hipStream_t st, const palz_scene_desc* scenes,
@@ -490,6 +563,7 @@ struct palz_batched {
     float* conv_params = nullptr;
     int* conv_iters = nullptr;
     uint32_t* list = nullptr;
+    float* thr = nullptr;  // per list position: the throughput after the occlusion rays
     uint32_t* scan_sums = nullptr;
     size_t cap_items = 0;
     palz_launch_config cfg[3] = {{256, 4, 256, 128}, {256, 2, 256, 128}, {128, 2, 256, 128}};
@@ -526,8 +600,10 @@ struct palz_batched {
     int ensure_items(size_t n) {
         if (n <= cap_items) return PALZ_OK;
         drop_graph();  // a captured graph holds the old buffers
-        for (void* p : {(void*)flags, (void*)conv_params, (void*)conv_iters, (void*)list, (void*)scan_sums})
+        for (void* p : {(void*)flags, (void*)conv_params, (void*)conv_iters, (void*)list, (void*)thr,
+                        (void*)scan_sums})
             if (p) PALZ_HIP(hipFree(p));
+        PALZ_HIP(hipMalloc(&thr, sizeof(float) * n));
         PALZ_HIP(hipMalloc(&flags, sizeof(uint32_t) * n));
         PALZ_HIP(hipMalloc(&conv_params, sizeof(float) * palz::kQueueStride * n));
         PALZ_HIP(hipMalloc(&conv_iters, sizeof(int) * n));
@@ -549,7 +625,8 @@ struct palz_batched {
             if (e) (void)hipEventDestroy(e);
         for (void* p : {(void*)paths, (void*)order, (void*)words, (void*)words_tmp, (void*)admissible,
                         (void*)ranges, (void*)ctr,
-                        (void*)flags, (void*)conv_params, (void*)conv_iters, (void*)list, (void*)scan_sums})
+                        (void*)flags, (void*)conv_params, (void*)conv_iters, (void*)list, (void*)thr,
+                        (void*)scan_sums})
             if (p) (void)hipFree(p);
     }
 
@@ -562,7 +639,7 @@ struct palz_batched {
     template <int K>
     void finish(const palz_launch_config& c, const palz_scene_desc* scenes, uint32_t L, int refine,
                 palz_path_result* results, double* refined, hipStream_t st) {
-        palz::launch_finish<K>(c, cus, st, scenes, paths, ranges, L, ctr, flags, list, conv_params, conv_iters,
+        palz::launch_finish<K>(c, cus, st, scenes, paths, ranges, L, ctr, flags, list, conv_params, conv_iters, thr,
                                refine, results, refined);
     }
     template <int K>

@@ -24,6 +24,7 @@ namespace {
 
 constexpr uint32_t kFlagAggregate = 1u;
 constexpr uint32_t kFlagInclusive = 2u;
+constexpr int kLookbackWindow = 8;  // predecessor tiles read per look-back step
 
 __device__ __forceinline__ uint64_t lb_pack(uint32_t ep, uint32_t flag, uint32_t count) {
     return ((uint64_t)((ep << 2) | flag) << 32) | (uint64_t)count;
@@ -96,6 +97,7 @@ __global__ void __launch_bounds__(kSortThreads)
                 uint64_t* __restrict__ lookback, uint32_t* __restrict__ ticket,
                 const uint32_t* __restrict__ epoch, const uint32_t* __restrict__ n_dev) {
     __shared__ uint32_t s_wave_hist[kSortWaves][256];
+    __shared__ uint32_t s_count[256];
     __shared__ int32_t s_local_start[256];
     __shared__ int32_t s_scatter_base[256];
     __shared__ uint32_t s_tot[kSortWaves];
@@ -110,6 +112,7 @@ __global__ void __launch_bounds__(kSortThreads)
     if (tid == 0) s_tile = atomicAdd(ticket, 1u);
 #pragma unroll
     for (int w = 0; w < kSortWaves; ++w) s_wave_hist[w][tid] = 0u;
+    s_count[tid] = 0u;
     __syncthreads();
 
     const uint32_t tile = s_tile;
@@ -132,62 +135,88 @@ __global__ void __launch_bounds__(kSortThreads)
         if constexpr (kValues) val[j] = idx < n ? vsrc[idx] : 0u;
     }
 
+    // The tile's digit counts first, so that they are published (and later
+    // tiles can look back past this one) while the keys are being ranked.
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const uint32_t idx = wave_base + (uint32_t)j * 64u + (uint32_t)lane;
+        if (idx < n) atomicAdd(&s_count[digit_of(key[j], shift, mask)], 1u);
+    }
+    __syncthreads();
+    const uint32_t d = (uint32_t)tid;  // one digit per thread from here
+    const uint32_t count = s_count[d];
+    uint64_t* my = lookback + (size_t)tile * 256u + d;
+    __hip_atomic_store(my, lb_pack(ep, tile == 0 ? kFlagInclusive : kFlagAggregate, count), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+
     // Wave64 multisplit: rank each key among equal digits, in key order.
     const uint64_t lt_mask = __lanemask_lt();
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
         const uint32_t idx = wave_base + (uint32_t)j * 64u + (uint32_t)lane;
         const bool valid = idx < n;
-        const uint32_t d = digit_of(key[j], shift, mask);
+        const uint32_t dj = digit_of(key[j], shift, mask);
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            const bool bit = (d >> b) & 1u;
+            const bool bit = (dj >> b) & 1u;
             const uint64_t bal = __ballot(bit);
             m &= bit ? bal : ~bal;
         }
         const int leader = valid ? (__ffsll((unsigned long long)m) - 1) : lane;
         uint32_t old = 0u;
         if (valid && lane == leader) {
-            old = s_wave_hist[wave][d];
-            s_wave_hist[wave][d] = old + (uint32_t)__popcll(m);
+            old = s_wave_hist[wave][dj];
+            s_wave_hist[wave][dj] = old + (uint32_t)__popcll(m);
         }
         old = (uint32_t)__shfl((int)old, leader, 64);
         rank[j] = old + (uint32_t)__popcll(m & lt_mask);
     }
     __syncthreads();
 
-    // Per digit (one per thread): exclusive prefix over waves, tile count.
-    const uint32_t d = (uint32_t)tid;
-    uint32_t count = 0u;
+    // Per digit: exclusive prefix over waves.
+    {
+        uint32_t acc = 0u;
 #pragma unroll
-    for (int w = 0; w < kSortWaves; ++w) {
-        const uint32_t c = s_wave_hist[w][d];
-        s_wave_hist[w][d] = count;
-        count += c;
+        for (int w = 0; w < kSortWaves; ++w) {
+            const uint32_t c = s_wave_hist[w][d];
+            s_wave_hist[w][d] = acc;
+            acc += c;
+        }
     }
 
     // Decoupled look-back over previous tiles for this digit.
-    uint64_t* my = lookback + (size_t)tile * 256u + d;
     uint32_t excl = 0u;
-    if (tile == 0) {
-        __hip_atomic_store(my, lb_pack(ep, kFlagInclusive, count), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        __hip_atomic_store(my, lb_pack(ep, kFlagAggregate, count), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+    if (tile != 0) {
+        // Walk back kLookbackWindow tiles per step: their entries are loaded
+        // together, then taken in order up to the nearest inclusive one (an
+        // unpublished entry ends the step; tiles before the first count as an
+        // inclusive zero).
         int p = (int)tile - 1;
-        while (p >= 0) {
-            const uint64_t e = __hip_atomic_load(lookback + (size_t)p * 256u + d, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t hi = (uint32_t)(e >> 32);
-            if ((hi >> 2) != ep) {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
+        for (;;) {
+            uint64_t e[kLookbackWindow];
+#pragma unroll
+            for (int w = 0; w < kLookbackWindow; ++w)
+                e[w] = p - w >= 0 ? __hip_atomic_load(lookback + (size_t)(p - w) * 256u + d, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                  : lb_pack(ep, kFlagInclusive, 0u);
+            int used = 0;
+            bool done = false, stalled = false;
+#pragma unroll
+            for (int w = 0; w < kLookbackWindow; ++w) {
+                if (done || stalled) continue;
+                const uint32_t hi = (uint32_t)(e[w] >> 32);
+                if ((hi >> 2) != ep) {
+                    stalled = true;
+                    continue;
+                }
+                excl += (uint32_t)e[w];
+                ++used;
+                done = (hi & 3u) == kFlagInclusive;
             }
-            excl += (uint32_t)e;
-            if ((hi & 3u) == kFlagInclusive) break;
-            --p;
+            if (done) break;
+            p -= used;
+            if (stalled) __builtin_amdgcn_s_sleep(1);
         }
         __hip_atomic_store(my, lb_pack(ep, kFlagInclusive, excl + count), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
