@@ -59,6 +59,21 @@ typedef struct {
     uint32_t context;
 } palz_internal_node;
 
+/* A two-level (BVH4) view of internal node i of an LBVH, for packet
+ * traversal: its two children with their boxes (as in the node) and, for each
+ * internal child c, that child's two children with their boxes at
+ * grand[2c + j] / grand_box[2c + j] (PALZ_NODE_INVALID below a leaf child).
+ * No counterpart in the reference: it holds what intersect_escape_link
+ * (minimum_lbvh.h:1078-1152) reads from three nodes, so that one load serves
+ * two levels.  176 B. */
+typedef struct {
+    palz_node_index child[2];
+    palz_node_index grand[4];
+    uint32_t pad_[2];
+    palz_aabb box[2];
+    palz_aabb grand_box[4];
+} palz_wide_node;
+
 /* Material (pk.h:67-72) and TriangleAttrib (pk.h:74-78): 40 B. */
 enum { PALZ_MATERIAL_DIFFUSE = 0, PALZ_MATERIAL_MIRROR = 1, PALZ_MATERIAL_DIELECTRIC = 2 };
 typedef struct {
@@ -147,11 +162,18 @@ int palz_lbvh_result(palz_lbvh* b, palz_internal_node** d_nodes, palz_node_index
  * exposed for parity checks. */
 int palz_lbvh_sorted(palz_lbvh* b, uint64_t** d_words, uint8_t** d_deltas);
 
+/* The BVH4 view of the last build (n-1 records, one per internal node;
+ * undefined for n = 1), computed on `stream` by this call and owned by the
+ * builder like the nodes (valid until the next build or destroy).  No
+ * counterpart in the reference; the batched solver's occlusion rays use it
+ * when a scene descriptor names it. */
+int palz_lbvh_widen(palz_lbvh* b, palz_wide_node** d_wide, void* stream);
+
 /* Stage timing (replaces the commented-out DeviceStopwatch of
  * minimum_lbvh.h:800-801, :884-885).  When enabled, each build records HIP
  * events on its stream after every stage; palz_lbvh_timings waits for the
- * last one and returns up to 8 durations in ms: [0] scene AABB, [1] Morton
- * keys + digit counts, [2..5] the four sort passes, [6] deltas, [7] build. */
+ * last one and returns up to 7 durations in ms: [0] scene AABB, [1] Morton
+ * keys + digit counts, [2..5] the four sort passes, [6] deltas and build. */
 int palz_lbvh_set_timing(palz_lbvh* b, int enable);
 int palz_lbvh_timings(palz_lbvh* b, float* ms, int max_stages, int* n_stages);
 
@@ -264,6 +286,9 @@ typedef struct {
     float radiance_clamp;    /* m_radianceClamp = 12 (pk.h:1688) */
     float cauchy_a;
     float cauchy_b;
+    /* Optional (NULL: the nodes alone): the BVH4 view of the same tree from
+     * palz_lbvh_widen, used by the batched solver's occlusion rays. */
+    const palz_wide_node* wide;
 } palz_scene_desc;
 
 /* Per (path, wavelength) outcome, item index = path * n_lambdas + l. */

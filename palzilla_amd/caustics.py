@@ -38,12 +38,12 @@ class SceneDesc(C.Structure):
                 ("points", C.c_void_p), ("lambdas", C.c_void_p), ("image", C.c_void_p),
                 ("n_points", C.c_uint32), ("n_lambdas", C.c_uint32), ("p_light", Float3),
                 ("light_intensity", C.c_float), ("radiance_clamp", C.c_float), ("cauchy_a", C.c_float),
-                ("cauchy_b", C.c_float)]
+                ("cauchy_b", C.c_float), ("wide", C.c_void_p)]
 
 
 assert C.sizeof(SceneView) == 40
 assert C.sizeof(PhotonParams) == 32
-assert C.sizeof(SceneDesc) == 92 or C.sizeof(SceneDesc) == 96
+assert C.sizeof(SceneDesc) == 104
 
 PHOTON_RECORD = np.dtype([("photon", "<u4"), ("event", "<u4"), ("tris", "<i4", 4), ("params", "<f4", 8),
                           ("p_final", "<f4", 3), ("throughput", "<f4")])
@@ -138,6 +138,7 @@ class GpuScene:
         self.d_image.zero(stream)
         self.bvh = BVHGPUBuilder()
         self.bvh.build(self.d_tris.ptr, scene.n_tris, 0, self.sorter, stream=stream, morton_bits=32)
+        self.wide = self.bvh.widen(stream)  # BVH4 view for the batched solver's occlusion rays
         self.table = TupleTable()
         self.d_records = None
         self.d_status = DeviceBuffer(16)
@@ -149,11 +150,13 @@ class GpuScene:
         return SceneView(self.d_tris.ptr, self.d_attribs.ptr, self.bvh.m_internals, self.bvh.m_rootNode,
                          self.scene.n_tris, 0)
 
-    def desc(self, image_ptr: int | None = None) -> SceneDesc:
+    def desc(self, image_ptr: int | None = None, wide: bool = True) -> SceneDesc:
+        """The scene's palz_scene_desc; wide=False leaves out the BVH4 view."""
         s = self.scene
         return SceneDesc(self.d_tris.ptr, self.d_attribs.ptr, self.bvh.m_internals, self.bvh.m_rootNode,
                          self.d_points.ptr, self.d_lambdas.ptr, image_ptr or self.d_image.ptr, self.n_points,
-                         self.n_lambdas, Float3(*s.p_light), LIGHT_INTENSITY, RADIANCE_CLAMP, s.cauchy_a, s.cauchy_b)
+                         self.n_lambdas, Float3(*s.p_light), LIGHT_INTENSITY, RADIANCE_CLAMP, s.cauchy_a, s.cauchy_b,
+                         self.wide if wide else None)
 
     def photon_params(self, iteration: int = 0) -> PhotonParams:
         s = self.scene

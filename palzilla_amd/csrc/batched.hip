@@ -364,8 +364,10 @@ __global__ void __launch_bounds__(256)
                     bool hit;
                     if (one_scene) {
                         const palz_scene_desc* us = scenes + scene0;
-                        hit = any_hit_packet(us->nodes, us->tris, *us->root, visible, seg[i].ro, seg[i].rd, ird,
-                                             1.0f, pstack);
+                        hit = us->wide ? any_hit_wide(us->nodes, us->wide, us->tris, *us->root, visible, seg[i].ro,
+                                                      seg[i].rd, ird, 1.0f, pstack)
+                                       : any_hit_packet(us->nodes, us->tris, *us->root, visible, seg[i].ro,
+                                                        seg[i].rd, ird, 1.0f, pstack);
                     } else {
                         hit = visible &&
                               any_hit_escape(sc->nodes, sc->tris, *sc->root, seg[i].ro, seg[i].rd, ird, 1.0f);

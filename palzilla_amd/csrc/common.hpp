@@ -28,6 +28,7 @@ static_assert(offsetof(palz_internal_node, aabbs) == 8, "InternalNode layout");
 static_assert(offsetof(palz_internal_node, parent) == 56, "InternalNode layout");
 static_assert(offsetof(palz_internal_node, context) == 60, "InternalNode layout");
 static_assert(sizeof(palz_triangle_attrib) == 40, "TriangleAttrib layout (pk.h:74)");
+static_assert(sizeof(palz_wide_node) == 176, "wide node layout");
 
 constexpr float kFltMax = 3.402823466e+38F;
 

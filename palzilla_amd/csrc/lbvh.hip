@@ -664,6 +664,33 @@ __global__ void __launch_bounds__(kGroupItems)
     }
 }
 
+// palz_wide_node of every internal node: its record and its internal
+// children's records, merged.
+__global__ void __launch_bounds__(256) k_widen(const palz_internal_node* __restrict__ nodes, uint32_t n_int,
+                                               palz_wide_node* __restrict__ wide) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_int) return;
+    const palz_internal_node nd = nodes[i];
+    palz_wide_node w;
+    w.pad_[0] = w.pad_[1] = 0u;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        w.child[c] = nd.children[c];
+        w.box[c] = nd.aabbs[c];
+        if (nd.children[c] & PALZ_NODE_LEAF_BIT) {
+            w.grand[2 * c] = w.grand[2 * c + 1] = PALZ_NODE_INVALID;
+            w.grand_box[2 * c] = w.grand_box[2 * c + 1] = palz_aabb{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+        } else {
+            const palz_internal_node cn = nodes[nd.children[c]];
+            w.grand[2 * c] = cn.children[0];
+            w.grand[2 * c + 1] = cn.children[1];
+            w.grand_box[2 * c] = cn.aabbs[0];
+            w.grand_box[2 * c + 1] = cn.aabbs[1];
+        }
+    }
+    wide[i] = w;
+}
+
 }  // namespace
 
 }  // namespace palz
@@ -679,6 +706,8 @@ struct palz_lbvh {
     palz::SubtreeRoot* staged[2] = {};   // per chunk / group, the subtree roots that go up (ping-pong)
     uint32_t* staged_count[2] = {};
     palz::BuildCtl* ctl = nullptr;
+    palz_wide_node* wide = nullptr;  // BVH4 view, on request
+    uint32_t wide_cap = 0;
     uint32_t capacity = 0;
     int n = 0;
     // Optional stage timing: events after each stage of the last build.
@@ -724,6 +753,7 @@ struct palz_lbvh {
         if (root) (void)hipFree(root);
         if (scene) (void)hipFree(scene);
         if (ctl) (void)hipFree(ctl);
+        if (wide) (void)hipFree(wide);
     }
 };
 
@@ -814,6 +844,26 @@ int palz_lbvh_result(palz_lbvh* b, palz_internal_node** d_nodes, palz_node_index
     if (d_root) *d_root = b->root;
     if (d_scene_aabb) *d_scene_aabb = b->scene;
     if (n_triangles) *n_triangles = b->n;
+    return PALZ_OK;
+}
+
+int palz_lbvh_widen(palz_lbvh* b, palz_wide_node** d_wide, void* stream) {
+    using namespace palz;
+    if (!b || !d_wide) return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_lbvh_widen: NULL");
+    if (b->n < 1) return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_lbvh_widen: nothing built yet");
+    const uint32_t n_int = (uint32_t)b->n - 1u;
+    const uint32_t want = std::max(1u, n_int);
+    if (want > b->wide_cap) {
+        if (b->wide) PALZ_HIP(hipFree(b->wide));
+        b->wide = nullptr;
+        PALZ_HIP(hipMalloc(&b->wide, sizeof(palz_wide_node) * want));
+        b->wide_cap = want;
+    }
+    if (n_int > 0u) {
+        k_widen<<<div_up(n_int, 256u), 256, 0, (hipStream_t)stream>>>(b->nodes, n_int, b->wide);
+        PALZ_LAUNCH_CHECK();
+    }
+    *d_wide = b->wide;
     return PALZ_OK;
 }
 

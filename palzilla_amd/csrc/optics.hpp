@@ -638,6 +638,22 @@ PALZ_HD Segment occlusion_segment(v3 from, v3 from_n, v3 to, v3 to_n) {
     return {from_safe, to_safe - from_safe};
 }
 
+// Loads of records at a wave-uniform address through the constant address
+// space, so that they become scalar loads into SGPRs (one load per wave
+// instead of one per lane).  The tree and the triangles are read-only while a
+// kernel runs.
+template <class T>
+__device__ __forceinline__ T load_uniform(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "dword records");
+    typedef const __attribute__((address_space(4))) uint32_t cword;
+    cword* q = (cword*)(p);
+    T r;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = q[i];
+    return r;
+}
+
 // Any-hit traversal of one occlusion ray per lane, run by the whole wave as a
 // packet: the wave walks the tree once, loading each node with scalar loads,
 // and every lane tests its own ray against the node's child boxes; a subtree
@@ -662,8 +678,8 @@ __device__ inline bool any_hit_packet(const palz_internal_node* __restrict__ nod
     uint64_t pending = __ballot(query);
     if (pending == 0ull) return false;
     auto test_leaf = [&](uint32_t leaf, uint64_t lanes) {
+        const palz_triangle tr = load_uniform(tris + (__builtin_amdgcn_readfirstlane(leaf) & ~PALZ_NODE_LEAF_BIT));
         if (lanes & me) {
-            const palz_triangle tr = ld_tri(tris, leaf & ~PALZ_NODE_LEAF_BIT);
             float t, u, v;
             v3 ng;
             if (intersect_tri(&t, &u, &v, &ng, 0.0f, t_max, ro, rd, mk(tr.vs[0]), mk(tr.vs[1]), mk(tr.vs[2])))
@@ -680,10 +696,9 @@ __device__ inline bool any_hit_packet(const palz_internal_node* __restrict__ nod
     uint32_t node = root;
     uint64_t mask = pending;
     for (;;) {
-        const uint32_t un = __builtin_amdgcn_readfirstlane(node);
-        const palz_internal_node* nd = nodes + un;
-        const uint32_t c0 = nd->children[0], c1 = nd->children[1];
-        const palz_aabb b0 = nd->aabbs[0], b1 = nd->aabbs[1];
+        const palz_internal_node nd = load_uniform(nodes + __builtin_amdgcn_readfirstlane(node));
+        const uint32_t c0 = nd.children[0], c1 = nd.children[1];
+        const palz_aabb b0 = nd.aabbs[0], b1 = nd.aabbs[1];
         bool h0 = false, h1 = false;
         if (mask & me) {
             float a, b;
@@ -732,6 +747,102 @@ __device__ inline bool any_hit_packet(const palz_internal_node* __restrict__ nod
             if (mask == 0ull) break;
         }
         if (pending == 0ull) break;
+    }
+    if (overflow && (pending & me)) hit = any_hit_escape(nodes, tris, root, ro, rd, ird, t_max);
+    return hit;
+}
+
+// any_hit_packet over the BVH4 view (palz_wide_node): one scalar load brings
+// a node's two child boxes and its internal children's child boxes, so the
+// wave's walk takes one dependent load per two levels.  A lane still tests a
+// grandchild's box only if it hit the child's box, so each lane tests exactly
+// the leaves any_hit_escape would, and the result is the same.
+__device__ inline bool any_hit_wide(const palz_internal_node* __restrict__ nodes,
+                                    const palz_wide_node* __restrict__ wide, const palz_triangle* __restrict__ tris,
+                                    uint32_t root, bool query, v3 ro, v3 rd, v3 ird, float t_max,
+                                    PacketStack* stack) {
+    const uint64_t me = 1ull << (threadIdx.x & 63);
+    bool hit = false;
+    uint64_t pending = __ballot(query);
+    if (pending == 0ull) return false;
+    auto test_leaf = [&](uint32_t leaf, uint64_t lanes) {
+        const palz_triangle tr = load_uniform(tris + (__builtin_amdgcn_readfirstlane(leaf) & ~PALZ_NODE_LEAF_BIT));
+        if (lanes & me) {
+            float t, u, v;
+            v3 ng;
+            if (intersect_tri(&t, &u, &v, &ng, 0.0f, t_max, ro, rd, mk(tr.vs[0]), mk(tr.vs[1]), mk(tr.vs[2])))
+                hit = true;
+        }
+        pending &= ~__ballot(hit);
+    };
+    auto box_hit = [&](const palz_aabb& b, uint64_t lanes) -> uint64_t {
+        bool h = false;
+        if (lanes & me) {
+            float a, c;
+            slabs(ro, ird, mk(b.lower), mk(b.upper), t_max, &a, &c);
+            h = a <= c;
+        }
+        return __ballot(h);
+    };
+    if (root & PALZ_NODE_LEAF_BIT) {
+        test_leaf(root, pending);
+        return hit;
+    }
+    int sp = 0;
+    bool overflow = false;
+    uint32_t node = root;
+    uint64_t mask = pending;
+    for (;;) {
+        const palz_wide_node wn = load_uniform(wide + __builtin_amdgcn_readfirstlane(node));
+        const palz_wide_node* w = &wn;
+        // the first internal grandchild hit is walked next, the others pushed
+        uint32_t first = PALZ_NODE_INVALID;
+        uint64_t first_mask = 0ull;
+        const bool leader = (threadIdx.x & 63) == __builtin_amdgcn_readfirstlane(threadIdx.x & 63);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const uint32_t ch = w->child[c];
+            const uint64_t mc = box_hit(w->box[c], mask) & pending;
+            if (mc == 0ull) continue;
+            if (ch & PALZ_NODE_LEAF_BIT) {
+                test_leaf(ch, mc);
+                continue;
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const uint32_t g = w->grand[2 * c + j];
+                const uint64_t mg = box_hit(w->grand_box[2 * c + j], mc & pending) & pending;
+                if (mg == 0ull) continue;
+                if (g & PALZ_NODE_LEAF_BIT) {
+                    test_leaf(g, mg);
+                } else if (first_mask == 0ull) {
+                    first = g;
+                    first_mask = mg;
+                } else if (sp == kPacketDepth) {
+                    overflow = true;
+                } else {
+                    if (leader) {
+                        stack->mask[sp] = mg;
+                        stack->node[sp] = g;
+                    }
+                    ++sp;
+                }
+            }
+        }
+        if (overflow || pending == 0ull) break;
+        if (first_mask != 0ull) {
+            node = first;
+            mask = first_mask;
+        } else {
+            mask = 0ull;
+            while (sp > 0 && mask == 0ull) {
+                --sp;
+                __builtin_amdgcn_wave_barrier();
+                node = stack->node[sp];
+                mask = stack->mask[sp] & pending;
+            }
+            if (mask == 0ull) break;
+        }
     }
     if (overflow && (pending & me)) hit = any_hit_escape(nodes, tris, root, ro, rd, ird, t_max);
     return hit;

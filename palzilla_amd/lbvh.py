@@ -62,6 +62,13 @@ class BVHGPUBuilder:
              C.c_uint32(build_option), int(morton_bits), C.c_void_p(sorter.handle), _sp(stream))
         self.m_nTriangles = int(n_triangles)
 
+    def widen(self, stream=None) -> int:
+        """Builds the BVH4 view (palz_wide_node per internal node) of the last
+        build on `stream`; returns its device pointer."""
+        w = C.c_void_p()
+        call("palz_lbvh_widen", C.c_void_p(self.handle), C.byref(w), _sp(stream))
+        return int(w.value or 0)
+
     def set_timing(self, enable: bool = True) -> None:
         call("palz_lbvh_set_timing", C.c_void_p(self.handle), int(enable))
 

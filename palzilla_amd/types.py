@@ -16,6 +16,11 @@ INTERNAL_NODE = np.dtype([("children", "<u4", 2), ("aabbs", AABB, 2), ("parent",
                           ("context", "<u4")])
 assert INTERNAL_NODE.itemsize == 64
 
+# palz_wide_node: the BVH4 view of one internal node, 176 B
+WIDE_NODE = np.dtype([("child", "<u4", 2), ("grand", "<u4", 4), ("pad", "<u4", 2), ("box", AABB, 2),
+                      ("grand_box", AABB, 4)])
+assert WIDE_NODE.itemsize == 176
+
 # TriangleAttrib (pk.h:74-78): int material + 3 x float3 shading normals, 40 B
 TRIANGLE_ATTRIB = np.dtype([("material", "<i4"), ("shading_normals", "<f4", (3, 3))])
 assert TRIANGLE_ATTRIB.itemsize == 40

@@ -8,7 +8,8 @@ import pytest
 import oracle
 from palzilla_amd import caustics as CA
 from palzilla_amd import scenes
-from palzilla_amd.device import DeviceBuffer
+from palzilla_amd import types as T
+from palzilla_amd.device import DeviceBuffer, download_ptr
 
 pytestmark = pytest.mark.gpu
 
@@ -61,6 +62,7 @@ class Run:
         self.paths = CA.download_paths(self.d_paths, self.n)
         self.L = s.lambdas.shape[0]
         self.d_scenes = CA.upload_scene_descs([self.g.desc()])
+        self.d_scenes_narrow = CA.upload_scene_descs([self.g.desc(wide=False)])
         self.bs = CA.BatchedSolver(self.g.sorter)
         self.bs.prepare(self.d_paths, self.n)
         self.order = self.bs.order()
@@ -68,14 +70,16 @@ class Run:
         self.d_ref = DeviceBuffer(max(self.n * self.L, 1) * 64)
         self._oracle = None
 
-    def solve(self, refine, graph=False):
+    def solve(self, refine, graph=False, wide=True):
         """One solve into the run's fixed result buffers (cleared first, eagerly),
-        so that repeated graph solves replay one captured graph."""
+        so that repeated graph solves replay one captured graph.  wide=False
+        solves with a descriptor without the BVH4 view."""
         n, L = self.n, self.L
         self.d_res.zero()
         self.d_ref.zero()
         self.g.clear_image()
-        self.bs.solve(self.d_scenes, 1, L, refine, self.d_res, self.d_ref, graph=graph)
+        self.bs.solve(self.d_scenes if wide else self.d_scenes_narrow, 1, L, refine, self.d_res, self.d_ref,
+                      graph=graph)
         res = _unsort(CA.download_results(self.d_res, n * L), self.order, L)
         ref = _unsort(self.d_ref.download(np.float64, n * L * 8).reshape(n * L, 8), self.order, L)
         return res, ref, self.g.image().reshape(-1)
@@ -128,6 +132,31 @@ def test_batched_matches_oracle_bit_exact(run):
     assert np.all(skipped["status"] == 0) and np.all(skipped["params"] == 0)
     assert (want["status"] == CA.PATH_CONTRIBUTES).sum() > 0
     assert np.allclose(img, want_img, rtol=1e-5, atol=1e-6)
+
+
+def test_wide_nodes_mirror_the_tree(run):
+    n_int = run.s.n_tris - 1
+    w = download_ptr(run.g.wide, T.WIDE_NODE, n_int)
+    nodes = run.nodes
+    assert np.array_equal(w["child"], nodes["children"])
+    assert w["box"].tobytes() == nodes["aabbs"].tobytes()
+    for c in range(2):
+        ch = nodes["children"][:, c]
+        internal = (ch & 0x80000000) == 0
+        idx = ch[internal]
+        assert np.array_equal(w["grand"][internal, 2 * c:2 * c + 2], nodes["children"][idx])
+        assert w["grand_box"][internal, 2 * c:2 * c + 2].tobytes() == nodes["aabbs"][idx].tobytes()
+        assert np.all(w["grand"][~internal, 2 * c:2 * c + 2] == 0x7FFFFFFF)
+
+
+def test_wide_walk_matches_node_walk(run):
+    # the occlusion rays walk the BVH4 view or the nodes: same results
+    res_w, _, img_w = run.solve(refine=CA.REFINE_NONE)
+    res_n, _, img_n = run.solve(refine=CA.REFINE_NONE, wide=False)
+    assert res_w.tobytes() == res_n.tobytes()
+    # equal contributions; the image sums them with float atomics in any order
+    assert np.allclose(img_w, img_n, rtol=1e-5, atol=1e-6)
+    assert (res_w["status"] == CA.PATH_CONTRIBUTES).sum() > 0
 
 
 def test_batched_stats(run):
