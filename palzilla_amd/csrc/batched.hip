@@ -301,8 +301,10 @@ __device__ __forceinline__ void refine_step(const float* prm, v3 pb, v3 pe, cons
 // takes a contiguous eighth of the positions, so that its L2 holds the nodes
 // and triangles of one part of the scene rather than of all of it; its waves
 // take kOccludeGrab positions at a time from that eighth's counter, then help
-// the other XCDs' eighths.
-constexpr uint32_t kOccludeGrab = 256;
+// the other XCDs' eighths.  A grab is 64 to 256 positions (one to four wave
+// passes): large enough to keep the counters cold on big lists, small enough
+// that a short list still spreads over every wave.
+constexpr uint32_t kOccludeGrabMax = 256;
 
 // Clears chain length K's per-XCD counters before its k_occlude (a kernel, not
 // a memset, so that it can be captured in a graph).
@@ -323,6 +325,8 @@ __global__ void __launch_bounds__(256)
     __shared__ PacketStack packet_stacks[256 / 64];
     PacketStack* pstack = &packet_stacks[threadIdx.x >> 6];
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = gridDim.x * (blockDim.x / 64u);
+    const uint32_t grab_size = 64u * min(kOccludeGrabMax / 64u, max(1u, (end - begin) / (waves * 64u * 8u)));
     const uint32_t per_xcd = div_up(end - begin, kXcds);
     const uint32_t home = blockIdx.x % kXcds;
     for (uint32_t k = 0; k < kXcds; ++k) {
@@ -331,10 +335,10 @@ __global__ void __launch_bounds__(256)
         const uint32_t r1 = begin + min((x + 1u) * per_xcd, end - begin);
         for (;;) {
             uint32_t grab = 0u;
-            if (lane == 0u) grab = atomicAdd(&ctr->occlude[K - 1][x], kOccludeGrab);
+            if (lane == 0u) grab = atomicAdd(&ctr->occlude[K - 1][x], grab_size);
             grab = __builtin_amdgcn_readfirstlane(grab);
             if (r0 + grab >= r1 || grab >= r1 - r0) break;
-            for (uint32_t j = 0; j < kOccludeGrab; j += 64u) {
+            for (uint32_t j = 0; j < grab_size; j += 64u) {
                 const uint32_t s = r0 + grab + j + lane;
                 if (__ballot(s < r1) == 0ull) break;
                 const bool in_range = s < r1;
