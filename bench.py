@@ -90,13 +90,21 @@ def NEWTON_BYTES(K: int) -> int:
     return 64 + 76 * K + 40 + 4 + 48
 
 
+def OCCLUDE_BYTES(K: int) -> int:
+    """Compulsory HBM bytes per converged item of k_occlude<K>: the list entry
+    (4), path record (64), FP32 solution (8K), K triangles and attributes (76
+    each), receiver point (40) and wavelength (4); written: the throughput (4).
+    BVH nodes and triangles read by the occlusion rays are not counted (the
+    BVH stays in cache)."""
+    return 4 + 64 + 8 * K + 76 * K + 40 + 4 + 4
+
+
 def FINISH_BYTES(K: int) -> int:
     """Compulsory HBM bytes per converged item of k_finish<K>: the list entry
-    (4), path record (64), FP32 solution (8K) and iteration count (4), K
-    triangles and attributes (76 each), receiver point (40), wavelength (4),
-    the 48-byte result record and the image word (4).  BVH nodes and triangles
-    read by the occlusion rays are not counted (the BVH stays in cache)."""
-    return 4 + 64 + 8 * K + 4 + 76 * K + 40 + 4 + 48 + 4
+    (4), path record (64), FP32 solution (8K), iteration count (4) and
+    throughput (4), K triangles and attributes (76 each), receiver point (40),
+    wavelength (4), the 48-byte result record and the image word (4)."""
+    return 4 + 64 + 8 * K + 4 + 4 + 76 * K + 40 + 4 + 48 + 4
 
 
 def profiled_traffic(workload: str, kernel: str):
@@ -294,8 +302,10 @@ def bench_c2(args) -> dict:
     for K in (1, 2, 3):
         row = stats[f"K{K}"]
         units[f"newton_k{K}"] = (row["items"], NEWTON_BYTES(K))
+        units[f"occlude_k{K}"] = (row["converged"], OCCLUDE_BYTES(K))
         units[f"finish_k{K}"] = (row["converged"], FINISH_BYTES(K))
     kernel_name = {f"newton_k{K}": f"k_newton<{K}>" for K in (1, 2, 3)}
+    kernel_name.update({f"occlude_k{K}": f"k_occlude<{K}>" for K in (1, 2, 3)})
     kernel_name.update({f"finish_k{K}": f"k_finish<{K}, false>" for K in (1, 2, 3)})
     kernel_name.update({"compact": "k_compact_scatter", "refine": "k_refine<K>", "cull": "k_cull_paths"})
     roof = {"bound": "hbm", "kernel": kernel_name[dominant], "kernel_ms": round(kern[dominant], 3)}

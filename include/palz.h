@@ -415,12 +415,13 @@ int palz_batched_stats(palz_batched* b, uint64_t* stats16, void* stream);
 /* Kernel timing of eager solves (graph launches are not timed): when
  * enabled, each palz_batched_solve records HIP events on its stream around
  * its kernels; palz_batched_timings waits for the last solve's events and
- * returns 9 durations in ms: Newton K=1, Newton K=2, Newton K=3, the
- * compaction of the converged items, finish K=1, finish K=2, finish K=3, the
+ * returns 12 durations in ms: Newton K=1, Newton K=2, Newton K=3, the
+ * compaction of the converged items, then for K = 1, 2, 3 the occlusion rays
+ * (contributableThroughput) and the finish (dA/dw, irradiance, results), the
  * FP64 refinement of all three (PALZ_REFINE_VERTICES; else ~0) and the
  * interval culling (when enabled; else ~0). */
 int palz_batched_set_timing(palz_batched* b, int enable);
-int palz_batched_timings(palz_batched* b, float* ms9);
+int palz_batched_timings(palz_batched* b, float* ms12);
 /* Per-tuple interval culling (palz_interval_cull) in front of the Newton
  * kernels: items of a culled path get status PALZ_PATH_CULLED without a
  * solve.  A culled path has no contributing solution, so the image and the

@@ -299,14 +299,14 @@ class BatchedSolver:
              int(refine), C.c_void_p(d_results.ptr if d_results else None),
              C.c_void_p(d_refined.ptr if d_refined else None), _sp(self.stream))
 
-    TIMED_KERNELS = ("newton_k1", "newton_k2", "newton_k3", "compact", "finish_k1", "finish_k2", "finish_k3",
-                     "refine", "cull")
+    TIMED_KERNELS = ("newton_k1", "newton_k2", "newton_k3", "compact", "occlude_k1", "finish_k1", "occlude_k2",
+                     "finish_k2", "occlude_k3", "finish_k3", "refine", "cull")
 
     def set_timing(self, enable: bool) -> None:
         call("palz_batched_set_timing", C.c_void_p(self.handle), int(bool(enable)))
 
     def timings(self) -> dict:
-        ms = (C.c_float * 9)()
+        ms = (C.c_float * len(self.TIMED_KERNELS))()
         call("palz_batched_timings", C.c_void_p(self.handle), ms)
         return dict(zip(self.TIMED_KERNELS, (float(v) for v in ms)))
 

@@ -529,13 +529,18 @@ void launch_newton(const palz_launch_config& c, int cus, hipStream_t st, const p
                                                           ctr, flags, cp, cit, admissible, results);
 }
 template <int K>
+void launch_occlude(const palz_launch_config& c, int cus, hipStream_t st, const palz_scene_desc* scenes,
+                    const palz_path* paths, const uint32_t* ranges, uint32_t L, Counters* ctr, const uint32_t* excl,
+                    const uint32_t* list, const float* cp, float* thr) {
+    k_clear_occlude<<<1, 64, 0, st>>>(ctr, K);
+    k_occlude<K><<<cus * 8, c.finish_block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, thr);
+}
+template <int K>
 void launch_finish(const palz_launch_config& c, int cus, hipStream_t st, const palz_scene_desc* scenes,
                    const palz_path* paths, const uint32_t* ranges, uint32_t L, Counters* ctr, const uint32_t* excl,
-                   const uint32_t* list, const float* cp, const int* cit, float* thr, int refine,
+                   const uint32_t* list, const float* cp, const int* cit, const float* thr, int refine,
                    palz_path_result* results, double* refined) {
     const dim3 grid(cus * 8), block(c.finish_block);
-    k_clear_occlude<<<1, 64, 0, st>>>(ctr, K);
-    k_occlude<K><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, thr);
     if (refine == PALZ_REFINE_ALL) {
         k_finish<K, true><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, cit, thr, results,
                                                   refined);
@@ -580,7 +585,7 @@ struct palz_batched {
     hipStream_t capture = nullptr;  // graphs are captured here (never the null stream)
     bool timing = false;
     bool timed_solves = false;
-    hipEvent_t ev[10] = {};
+    hipEvent_t ev[13] = {};
     bool cull = false;
     uint8_t* admissible = nullptr;  // per sorted path, when culling
     struct Key {
@@ -643,6 +648,10 @@ struct palz_batched {
                                cull ? admissible : nullptr, results);
     }
     template <int K>
+    void occlude(const palz_launch_config& c, const palz_scene_desc* scenes, uint32_t L, hipStream_t st) {
+        palz::launch_occlude<K>(c, cus, st, scenes, paths, ranges, L, ctr, flags, list, conv_params, thr);
+    }
+    template <int K>
     void finish(const palz_launch_config& c, const palz_scene_desc* scenes, uint32_t L, int refine,
                 palz_path_result* results, double* refined, hipStream_t st) {
         palz::launch_finish<K>(c, cus, st, scenes, paths, ranges, L, ctr, flags, list, conv_params, conv_iters, thr,
@@ -679,10 +688,16 @@ struct palz_batched {
         if (t) PALZ_HIP(hipEventRecord(ev[3], st));
         compact(L, results, st);
         if (t) PALZ_HIP(hipEventRecord(ev[4], st));
+        occlude<1>(cfg[0], scenes, L, st);
+        if (t) PALZ_HIP(hipEventRecord(ev[10], st));
         finish<1>(cfg[0], scenes, L, refine, results, refined, st);
         if (t) PALZ_HIP(hipEventRecord(ev[5], st));
+        occlude<2>(cfg[1], scenes, L, st);
+        if (t) PALZ_HIP(hipEventRecord(ev[11], st));
         finish<2>(cfg[1], scenes, L, refine, results, refined, st);
         if (t) PALZ_HIP(hipEventRecord(ev[6], st));
+        occlude<3>(cfg[2], scenes, L, st);
+        if (t) PALZ_HIP(hipEventRecord(ev[12], st));
         finish<3>(cfg[2], scenes, L, refine, results, refined, st);
         if (t) PALZ_HIP(hipEventRecord(ev[7], st));
         refine_vertices<1>(cfg[0], scenes, L, refine, results, refined, st);
@@ -837,14 +852,16 @@ int palz_batched_set_timing(palz_batched* b, int enable) {
     return PALZ_OK;
 }
 
-int palz_batched_timings(palz_batched* b, float* ms8) {  // 9 entries
-    if (!b || !ms8) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_timings: NULL");
+int palz_batched_timings(palz_batched* b, float* ms12) {
+    if (!b || !ms12) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_timings: NULL");
     if (!b->timing || !b->timed_solves)
         return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_timings: no timed solve since timing was enabled");
-    PALZ_HIP(hipEventSynchronize(b->ev[8]));
-    PALZ_HIP(hipEventElapsedTime(&ms8[0], b->ev[9], b->ev[1]));
-    for (int i = 1; i < 8; ++i) PALZ_HIP(hipEventElapsedTime(&ms8[i], b->ev[i], b->ev[i + 1]));
-    PALZ_HIP(hipEventElapsedTime(&ms8[8], b->ev[0], b->ev[9]));
+    hipEvent_t* ev = b->ev;
+    PALZ_HIP(hipEventSynchronize(ev[8]));
+    // (from, to) event pairs in the order of the reported durations
+    const int pairs[12][2] = {{9, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 10}, {10, 5},
+                              {5, 11}, {11, 6}, {6, 12}, {12, 7}, {7, 8}, {0, 9}};
+    for (int i = 0; i < 12; ++i) PALZ_HIP(hipEventElapsedTime(&ms12[i], ev[pairs[i][0]], ev[pairs[i][1]]));
     return PALZ_OK;
 }
 
@@ -932,6 +949,9 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
             palz_launch_config c = best;
             c.finish_block = fb;
             (void)hipEventRecord(e0, st);
+            if (K == 1) b->occlude<1>(c, d_scenes, n_lambdas, st);
+            if (K == 2) b->occlude<2>(c, d_scenes, n_lambdas, st);
+            if (K == 3) b->occlude<3>(c, d_scenes, n_lambdas, st);
             if (K == 1) b->finish<1>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
             if (K == 2) b->finish<2>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
             if (K == 3) b->finish<3>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
