@@ -313,7 +313,7 @@ __global__ void k_clear_occlude(Counters* c, int K) {
 }
 
 template <int K>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 8)  // 8 waves per SIMD: latency-bound, few live registers
     k_occlude(const palz_scene_desc* __restrict__ scenes, const palz_path* __restrict__ paths,
               const uint32_t* __restrict__ ranges, uint32_t L, Counters* __restrict__ ctr,
               const uint32_t* __restrict__ excl, const uint32_t* __restrict__ list,
