@@ -126,9 +126,11 @@ __device__ __forceinline__ void wave_add_stat(unsigned long long* dst, unsigned 
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
 }
 
-// Persistent Newton iterations of every item of chain length K.
+// Persistent Newton iterations of every item of chain length K.  For K = 2
+// the register budget is held at 168 so that three waves share a SIMD
+// (K = 3 would spill).
 template <int K>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, K == 2 ? 3 : 1)
     k_newton(const palz_scene_desc* __restrict__ scenes, uint32_t n_scenes, const palz_path* __restrict__ paths,
              const uint32_t* __restrict__ ranges, uint32_t L, uint32_t chunk, Counters* __restrict__ ctr,
              uint32_t* __restrict__ flags, float* __restrict__ conv_params, int* __restrict__ conv_iters,
