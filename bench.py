@@ -47,11 +47,13 @@ STAGE_BYTES_PER_TRI = {
     "scene_aabb": 36.0, "morton_hist": 44.0, "sort_pass0": 16.0, "sort_pass1": 16.0,
     "sort_pass2": 16.0, "sort_pass3": 16.0,
     "build": 8.0 + 36.0 + 1.0 + 64.0,  # sorted word + triangle + delta + internal node
+    "build_levels": 0.0,  # the levels above the leaves: a few per cent of the nodes
 }
 LBVH_BYTES_PER_TRI = sum(STAGE_BYTES_PER_TRI.values())  # 253 B/tri
 LBVH_KERNEL = {"scene_aabb": "k_scene_aabb", "morton_hist": "k_morton_hist", "sort_pass0": "k_sort_pass<false>",
                "sort_pass1": "k_sort_pass<false>", "sort_pass2": "k_sort_pass<false>",
-               "sort_pass3": "k_sort_pass<false>", "build": "k_build"}
+               "sort_pass3": "k_sort_pass<false>", "build": "k_build",
+               "build_levels": "k_build_level"}
 
 
 def log(msg: str) -> None:

@@ -172,8 +172,9 @@ int palz_lbvh_widen(palz_lbvh* b, palz_wide_node** d_wide, void* stream);
 /* Stage timing (replaces the commented-out DeviceStopwatch of
  * minimum_lbvh.h:800-801, :884-885).  When enabled, each build records HIP
  * events on its stream after every stage; palz_lbvh_timings waits for the
- * last one and returns up to 7 durations in ms: [0] scene AABB, [1] Morton
- * keys + digit counts, [2..5] the four sort passes, [6] deltas and build. */
+ * last one and returns up to 8 durations in ms: [0] scene AABB, [1] Morton
+ * keys + digit counts, [2..5] the four sort passes, [6] deltas and the leaf
+ * level of the build, [7] the levels above it. */
 int palz_lbvh_set_timing(palz_lbvh* b, int enable);
 int palz_lbvh_timings(palz_lbvh* b, float* ms, int max_stages, int* n_stages);
 

@@ -711,7 +711,7 @@ struct palz_lbvh {
     uint32_t capacity = 0;
     int n = 0;
     // Optional stage timing: events after each stage of the last build.
-    static constexpr int kStages = 7;
+    static constexpr int kStages = 8;
     bool timing = false;
     hipEvent_t ev[kStages + 1] = {};
 
@@ -824,6 +824,7 @@ int palz_lbvh_build(palz_lbvh* b, const palz_triangle* d_tris, int n, uint32_t b
     uint32_t chunks = div_up(un, (uint32_t)kBuildChunk);
     k_build<<<chunks, kBuildChunk, 0, st>>>(d_tris, b->words, un, b->deltas, b->nodes, b->root, b->arrival, b->ctl,
                                             b->staged[0], chunks > 1u ? b->staged_count[0] : nullptr);
+    if (ev) PALZ_HIP(hipEventRecord(ev[7], st));
     for (int cur = 0; chunks > 1u; cur ^= 1) {
         const uint32_t groups = div_up(chunks, (uint32_t)kGroupChunks);
         k_build_level<<<groups, kGroupItems, 0, st>>>(b->staged[cur], b->staged_count[cur], chunks, un, b->deltas,
@@ -832,7 +833,7 @@ int palz_lbvh_build(palz_lbvh* b, const palz_triangle* d_tris, int n, uint32_t b
         chunks = groups;
     }
     PALZ_LAUNCH_CHECK();
-    if (ev) PALZ_HIP(hipEventRecord(ev[7], st));
+    if (ev) PALZ_HIP(hipEventRecord(ev[8], st));
     PALZ_LAUNCH_CHECK();
     return PALZ_OK;
 }

@@ -73,7 +73,7 @@ class BVHGPUBuilder:
         call("palz_lbvh_set_timing", C.c_void_p(self.handle), int(enable))
 
     STAGES = ("scene_aabb", "morton_hist", "sort_pass0", "sort_pass1", "sort_pass2",
-              "sort_pass3", "build")
+              "sort_pass3", "build", "build_levels")
 
     def timings(self) -> dict:
         """Per-stage kernel durations (ms) of the last build, from HIP events."""
