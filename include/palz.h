@@ -169,6 +169,21 @@ int palz_lbvh_sorted(palz_lbvh* b, uint64_t** d_words, uint8_t** d_deltas);
  * when a scene descriptor names it. */
 int palz_lbvh_widen(palz_lbvh* b, palz_wide_node** d_wide, void* stream);
 
+/* InternalNormalBound (pk.h:1705-1709): per internal node, the AABB of the
+ * shading normals of every triangle under each child, and the arrival
+ * counter of the bottom-up pass (2 for every node afterwards).  52 B. */
+typedef struct {
+    palz_aabb normal_bounds[2];
+    uint32_t counter;
+} palz_normal_bound;
+
+/* The bottom-up normal-bound propagation of main.cpp:248-295 over the last
+ * build's tree: d_attribs[t] is triangle t's TriangleAttrib; d_out receives
+ * n-1 records (nothing for n = 1).  Boxes take the canonical min/max, so they
+ * do not depend on the order the subtrees arrive in. */
+int palz_lbvh_normal_bounds(palz_lbvh* b, const palz_triangle_attrib* d_attribs, palz_normal_bound* d_out,
+                            void* stream);
+
 /* Stage timing (replaces the commented-out DeviceStopwatch of
  * minimum_lbvh.h:800-801, :884-885).  When enabled, each build records HIP
  * events on its stream after every stage; palz_lbvh_timings waits for the

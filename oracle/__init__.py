@@ -26,6 +26,7 @@ _SIGS = {
     "orc_lbvh": (None, [_P, C.c_int, C.c_uint32, C.c_int, _P, _P, _P, _P, _P]),
     "orc_lbvh_topdown": (None, [_P, C.c_int, _P, _P, _P, _P]),
     "orc_validate_lbvh": (C.c_int, [_P, C.c_uint32, _P]),
+    "orc_normal_bounds": (None, [_P, C.c_int, _P, _P]),
     "orc_sobol2d": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "orc_photon_trace": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, _P]),
     "orc_lookup_paths": (C.c_uint32, [_P, C.c_uint32, C.c_float, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P,
@@ -120,6 +121,15 @@ def build_lbvh(tris: np.ndarray, sorted_words: np.ndarray, deltas: np.ndarray):
     lib().orc_build_lbvh(_ptr(tris), n, _ptr(np.ascontiguousarray(sorted_words, dtype=np.uint64)),
                          _ptr(np.ascontiguousarray(deltas, dtype=np.uint8)), _ptr(nodes), _ptr(root))
     return nodes[: n - 1], int(root[0])
+
+
+def normal_bounds(nodes: np.ndarray, attribs: np.ndarray) -> np.ndarray:
+    """InternalNormalBound records of the tree (main.cpp:248-295), serially."""
+    from palzilla_amd import types as T
+    n_int = nodes.shape[0]
+    out = np.zeros(max(n_int, 1), dtype=T.NORMAL_BOUND)
+    lib().orc_normal_bounds(_ptr(np.ascontiguousarray(nodes)), n_int, _ptr(np.ascontiguousarray(attribs)), _ptr(out))
+    return out[:n_int]
 
 
 def validate_lbvh(nodes: np.ndarray, root: int, deltas: np.ndarray) -> bool:

@@ -250,6 +250,48 @@ void orc_lbvh_topdown(const palz_triangle* tris, int n, const uint64_t* sorted_w
     if (!(r & kLeaf)) nodes[r].parent = kInvalid;
 }
 
+// The InternalNormalBound pass (main.cpp:248-295), serially: for each internal
+// node's leaf children, climb by parent links, storing the subtree's box of
+// shading normals into the parent's slot; the second arrival merges and goes
+// on.
+void orc_normal_bounds(const palz_internal_node* nodes, int n_int, const palz_triangle_attrib* attribs,
+                       palz_normal_bound* out) {
+    for (int i = 0; i < n_int; ++i) out[i].counter = 0u;
+    for (int i = 0; i < n_int; ++i) {
+        for (int j = 0; j < 2; ++j) {
+            uint32_t me = nodes[i].children[j];
+            if (!(me & kLeaf)) continue;
+            const palz_triangle_attrib& at = attribs[me & ~kLeaf];
+            float lo[3] = {+3.40282347e+38f, +3.40282347e+38f, +3.40282347e+38f};
+            float hi[3] = {-3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f};
+            for (int k = 0; k < 3; ++k) {
+                const float p[3] = {at.shading_normals[k].x, at.shading_normals[k].y, at.shading_normals[k].z};
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = box_min(lo[a], p[a]);
+                    hi[a] = box_max(hi[a], p[a]);
+                }
+            }
+            uint32_t parent = (uint32_t)i;
+            while (parent != kInvalid) {
+                const int c = nodes[parent].children[0] == me ? 0 : 1;
+                palz_aabb& dst = out[parent].normal_bounds[c];
+                dst.lower = {lo[0], lo[1], lo[2]};
+                dst.upper = {hi[0], hi[1], hi[2]};
+                if (out[parent].counter++ == 0u) break;
+                const palz_aabb& sb = out[parent].normal_bounds[c ^ 1];
+                const float slo[3] = {sb.lower.x, sb.lower.y, sb.lower.z};
+                const float shi[3] = {sb.upper.x, sb.upper.y, sb.upper.z};
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = box_min(lo[a], slo[a]);
+                    hi[a] = box_max(hi[a], shi[a]);
+                }
+                me = parent;
+                parent = nodes[me].parent;
+            }
+        }
+    }
+}
+
 // 1 when deltas never increase going down the tree (validate_lbvh).
 int orc_validate_lbvh(const palz_internal_node* nodes, uint32_t root, const uint8_t* deltas) {
     return validate(root, nodes, deltas, 255u) ? 1 : 0;

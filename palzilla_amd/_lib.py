@@ -55,6 +55,7 @@ _SIGS = {
                                    C.POINTER(C.c_int)]),
     "palz_lbvh_sorted": (C.c_int, [C.c_void_p, c_void_pp, c_void_pp]),
     "palz_lbvh_widen": (C.c_int, [C.c_void_p, c_void_pp, C.c_void_p]),
+    "palz_lbvh_normal_bounds": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "palz_photon_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "palz_tuple_table_create": (C.c_int, [c_void_pp]),
     "palz_tuple_table_destroy": (C.c_int, [C.c_void_p]),

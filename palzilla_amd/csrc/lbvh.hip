@@ -691,6 +691,71 @@ __global__ void __launch_bounds__(256) k_widen(const palz_internal_node* __restr
     wide[i] = w;
 }
 
+// InternalNormalBound pass (main.cpp:248-295): the counters start at zero.
+__global__ void __launch_bounds__(256) k_normal_bounds_clear(palz_normal_bound* __restrict__ out, uint32_t n_int) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_int) out[i].counter = 0u;
+}
+
+// One thread per internal node climbs from each of its leaf children towards
+// the root: it stores the subtree's normal box into the parent's slot for
+// that child, and the second of the two arrivals at a node merges the
+// sibling's box and goes on.  The box travels through memory written through
+// to the device-coherent level (32-bit coherent stores: the 52-byte records
+// are only 4-byte aligned), a vmcnt wait orders it before the arrival count,
+// and the second arriver reads it back with coherent loads.
+__device__ __forceinline__ void store_nbox(palz_aabb* dst, v3 lo, v3 hi) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(dst);
+    const float f[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        __hip_atomic_store(w + k, __float_as_uint(f[k]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void load_nbox(palz_aabb* src, v3* lo, v3* hi) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(src);
+    float f[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        f[k] = __uint_as_float(__hip_atomic_load(w + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    *lo = mk(f[0], f[1], f[2]);
+    *hi = mk(f[3], f[4], f[5]);
+}
+
+__global__ void __launch_bounds__(256) k_normal_bounds(const palz_internal_node* __restrict__ nodes, uint32_t n_int,
+                                                       const palz_triangle_attrib* __restrict__ attribs,
+                                                       palz_normal_bound* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_int) return;
+    for (int j = 0; j < 2; ++j) {
+        palz_node_index me = nodes[i].children[j];
+        if (!(me & PALZ_NODE_LEAF_BIT)) continue;
+        const palz_triangle_attrib& at = attribs[me & ~PALZ_NODE_LEAF_BIT];
+        v3 lo = mk(at.shading_normals[0]), hi = lo;
+        lo = cmin(lo, mk(at.shading_normals[1]));
+        hi = cmax(hi, mk(at.shading_normals[1]));
+        lo = cmin(lo, mk(at.shading_normals[2]));
+        hi = cmax(hi, mk(at.shading_normals[2]));
+        uint32_t parent = i;
+        while (parent != PALZ_NODE_INVALID) {
+            const int c = nodes[parent].children[0] == me ? 0 : 1;
+            store_nbox(&out[parent].normal_bounds[c], lo, hi);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0);  // the coherent box stores have completed
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            const uint32_t v = __hip_atomic_fetch_add(&out[parent].counter, 1u, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            if (v == 0u) break;
+            v3 slo, shi;
+            load_nbox(&out[parent].normal_bounds[c ^ 1], &slo, &shi);
+            lo = cmin(lo, slo);
+            hi = cmax(hi, shi);
+            me = parent;
+            parent = nodes[me].parent;
+        }
+    }
+}
+
 }  // namespace
 
 }  // namespace palz
@@ -865,6 +930,20 @@ int palz_lbvh_widen(palz_lbvh* b, palz_wide_node** d_wide, void* stream) {
         PALZ_LAUNCH_CHECK();
     }
     *d_wide = b->wide;
+    return PALZ_OK;
+}
+
+int palz_lbvh_normal_bounds(palz_lbvh* b, const palz_triangle_attrib* d_attribs, palz_normal_bound* d_out,
+                            void* stream) {
+    using namespace palz;
+    if (!b || !d_attribs || !d_out) return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_lbvh_normal_bounds: NULL");
+    if (b->n < 1) return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_lbvh_normal_bounds: nothing built yet");
+    const uint32_t n_int = (uint32_t)b->n - 1u;
+    if (n_int == 0u) return PALZ_OK;
+    hipStream_t st = (hipStream_t)stream;
+    k_normal_bounds_clear<<<div_up(n_int, 256u), 256, 0, st>>>(d_out, n_int);
+    k_normal_bounds<<<div_up(n_int, 256u), 256, 0, st>>>(b->nodes, n_int, d_attribs, d_out);
+    PALZ_LAUNCH_CHECK();
     return PALZ_OK;
 }
 

@@ -69,6 +69,16 @@ class BVHGPUBuilder:
         call("palz_lbvh_widen", C.c_void_p(self.handle), C.byref(w), _sp(stream))
         return int(w.value or 0)
 
+    def normal_bounds(self, attribs: int, stream=None) -> DeviceBuffer:
+        """InternalNormalBound per internal node of the last build (the
+        bottom-up pass of main.cpp:248-295), from the device TriangleAttrib
+        array `attribs`; returns the n-1 records."""
+        n_int = max(self.m_nTriangles - 1, 0)
+        out = DeviceBuffer(max(n_int, 1) * T.NORMAL_BOUND.itemsize)
+        call("palz_lbvh_normal_bounds", C.c_void_p(self.handle), C.c_void_p(attribs), C.c_void_p(out.ptr),
+             _sp(stream))
+        return out
+
     def set_timing(self, enable: bool = True) -> None:
         call("palz_lbvh_set_timing", C.c_void_p(self.handle), int(enable))
 

@@ -21,6 +21,10 @@ WIDE_NODE = np.dtype([("child", "<u4", 2), ("grand", "<u4", 4), ("pad", "<u4", 2
                       ("grand_box", AABB, 4)])
 assert WIDE_NODE.itemsize == 176
 
+# InternalNormalBound (pk.h:1705-1709): 2 AABBs of shading normals + arrival counter, 52 B
+NORMAL_BOUND = np.dtype([("normal_bounds", AABB, 2), ("counter", "<u4")])
+assert NORMAL_BOUND.itemsize == 52
+
 # TriangleAttrib (pk.h:74-78): int material + 3 x float3 shading normals, 40 B
 TRIANGLE_ATTRIB = np.dtype([("material", "<i4"), ("shading_normals", "<f4", (3, 3))])
 assert TRIANGLE_ATTRIB.itemsize == 40

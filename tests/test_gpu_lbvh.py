@@ -165,3 +165,27 @@ def test_lbvh_reference_mesh_bit_exact(sorter, bits):
     assert np.array_equal(g_words, words) and np.array_equal(g_deltas, deltas)
     assert b.download_root() == root
     assert b.download_nodes().tobytes() == nodes.tobytes()
+
+
+@pytest.mark.parametrize("n,option,run", [(2, 0, 0), (3, 0, 0), (257, 0, 0), (4097, 1, 0), (100_003, 1, 0),
+                                          (40_000, 0, 600)])
+def test_normal_bounds_bit_exact(sorter, n, option, run):
+    # InternalNormalBound (main.cpp:248-295) on the GPU tree, against the
+    # oracle's serial pass; `run` identical triangles exercise the
+    # device-wide part of the build
+    print(f"normal bounds n={n} option={option} run={run}", flush=True)
+    tris = scenes.triangle_soup(n, seed=n + 5)
+    if run:
+        tris[n // 3:n // 3 + run] = tris[n // 3]
+    b = BVHGPUBuilder()
+    dt = DeviceBuffer.from_array(tris)
+    b.build(dt.ptr, n, option, sorter, morton_bits=32)
+    rng = np.random.default_rng(n)
+    attribs = np.zeros(n, dtype=T.TRIANGLE_ATTRIB)
+    attribs["material"] = 2
+    attribs["shading_normals"] = rng.normal(size=(n, 3, 3)).astype(np.float32)
+    d_attr = DeviceBuffer.from_array(attribs)
+    got = b.normal_bounds(d_attr.ptr).download(T.NORMAL_BOUND, n - 1)
+    want = oracle.normal_bounds(b.download_nodes(), attribs)
+    assert got.tobytes() == want.tobytes()
+    assert np.all(got["counter"] == 2)

@@ -173,3 +173,39 @@ def test_reference_mesh_fixture(bits):
     td_nodes, td_root = oracle.lbvh_topdown(tris, words, deltas)
     assert td_root == root and td_nodes.tobytes() == nodes.tobytes()
     _check_tree(tris, nodes, root, words, deltas)
+
+
+def _subtree_leaves(nodes, node):
+    out, stack = [], [node]
+    while stack:
+        x = int(stack.pop())
+        if x & 0x80000000:
+            out.append(x & 0x7FFFFFFF)
+        else:
+            stack.extend(nodes["children"][x])
+    return out
+
+
+def random_attribs(n, seed):
+    rng = np.random.default_rng(seed)
+    a = np.zeros(n, dtype=T.TRIANGLE_ATTRIB)
+    a["material"] = 2
+    a["shading_normals"] = rng.normal(size=(n, 3, 3)).astype(np.float32)
+    return a
+
+
+@pytest.mark.parametrize("n", [2, 3, 17, 1000])
+def test_normal_bounds_brute_force(n):
+    # every slot holds the box of the shading normals under that child: the
+    # serial restatement of main.cpp:248-295 against a direct subtree walk
+    tris = scenes.triangle_soup(n, seed=n + 11)
+    nodes, root, *_ = oracle.lbvh(tris, option=T.BUILD_OPTION_USE_NORMAL, bits=32)
+    attribs = random_attribs(n, n)
+    nb = oracle.normal_bounds(nodes, attribs)
+    assert np.all(nb["counter"] == 2)
+    for i in range(n - 1):
+        for c in range(2):
+            sn = attribs["shading_normals"][_subtree_leaves(nodes, nodes["children"][i][c])].reshape(-1, 3)
+            box = nb["normal_bounds"][i][c]
+            assert np.array_equal(box["lower"], sn.min(axis=0)), (i, c)
+            assert np.array_equal(box["upper"], sn.max(axis=0)), (i, c)
