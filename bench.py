@@ -237,6 +237,23 @@ def bench_c2(args) -> dict:
                "step_ms_with_culling": round(sum(cull_kern.values()), 3),
                "contributing_items_with_culling": sum(cull_stats[f"K{k}"]["contributing"] for k in (1, 2, 3))}
 
+    # spectral resolve of the receiver images (CIE fit, XYZ -> sRGB, pack):
+    # reads L images and the receiver points, writes float4 + RGBA8 per point
+    w_l = np.full(L, (float(s.lambdas.max()) - float(s.lambdas.min())) / max(L - 1, 1), dtype=np.float32)
+    d_w = DeviceBuffer.from_array(w_l)
+    n_pts = s.points.shape[0]
+    CA.spectral_resolve(g.d_image, g.d_lambdas, d_w, g.d_points, n_pts, L, stream=st)
+    st.sync()
+    t0.record(st)
+    for _ in range(10):
+        CA.spectral_resolve(g.d_image, g.d_lambdas, d_w, g.d_points, n_pts, L, stream=st)
+    t1.record(st)
+    st.sync()
+    ms_resolve = t0.elapsed_ms(t1) / 10
+    resolve_bytes = n_pts * (4 * L + 40 + 16 + 4)
+    resolve = {"ms": round(ms_resolve, 4), "bytes": resolve_bytes,
+               "GBps": round(resolve_bytes / (ms_resolve * 1e-3) / 1e9, 1)}
+
     # the same solve as one hipGraph launch
     g.clear_image()
     bs.solve(d_scenes, 1, L, CA.REFINE_VERTICES, d_res, graph=True)
@@ -351,6 +368,7 @@ def bench_c2(args) -> dict:
                   "batched_over_plain_time": round(ms_plain / ms, 2),
                   "image_rel_rmse_batched_vs_plain": img_rmse},
         "interval_culling": culling,
+        "spectral_resolve": resolve,
         "vertex_error": sample["vertex"],
         "parity_sample": sample["parity"],
         "pipeline_ms": {k: round(v, 3) for k, v in pipeline_ms.items()},
@@ -604,6 +622,20 @@ def bench_lbvh(args) -> dict:
     keys = (words & np.uint64(0xFFFFFFFF)).astype(np.int64)
     assert np.all(np.diff(keys) >= 0), "sorted keys are not sorted"
     assert np.array_equal(np.sort(words >> np.uint64(32)), np.arange(n, dtype=np.uint64))
+    # the smaller sizes of the same workload (1M and 4M triangles): time per build
+    by_size = {}
+    for m in (1 << 20, 1 << 22):
+        if m >= n:
+            continue
+        for _ in range(args.warmup):
+            b.build(d_tris.ptr, m, 0, sorter, stream=st, morton_bits=30)
+        e0.record(st)
+        for _ in range(args.steps):
+            b.build(d_tris.ptr, m, 0, sorter, stream=st, morton_bits=30)
+        e1.record(st)
+        st.sync()
+        ms_m = e0.elapsed_ms(e1) / args.steps
+        by_size[str(m)] = {"ms_per_build": round(ms_m, 4), "Mtri_per_s": round(m / (ms_m * 1e-3) / 1e6, 1)}
     achieved = LBVH_BYTES_PER_TRI * n / (ms * 1e-3) / 1e9
     out = {"workload": "lbvh+onesweep, 16M-triangle soup (30-bit Morton keys)", "n_triangles": n,
            "value": round(n / (ms * 1e-3) / 1e6, 2), "unit": "Mtri/s", "ms_per_step": round(ms, 4),
@@ -612,7 +644,7 @@ def bench_lbvh(args) -> dict:
            "hbm_frac": round(achieved / HBM_PEAK_GBPS, 4),
            "stage_GBps": {k: round(STAGE_BYTES_PER_TRI[k] * n / (v * 1e-3) / 1e9, 1)
                           for k, v in stage_ms.items() if v > 0 and STAGE_BYTES_PER_TRI[k] > 0},
-           "copy_kernel_GBps": round(measure_copy_bandwidth(), 1)}
+           "copy_kernel_GBps": round(measure_copy_bandwidth(), 1), "smaller_sizes": by_size}
     if not args.no_cpu:
         import oracle
         sample = min(n, args.cpu_sample)

@@ -36,6 +36,7 @@ enum {
 
 /* float3: 12 B, no padding (helper_math.h float3). */
 typedef struct { float x, y, z; } palz_float3;
+typedef struct { float x, y, z, w; } palz_float4;
 
 /* minimum_lbvh::Triangle (minimum_lbvh.h:207-210): 36 B. */
 typedef struct { palz_float3 vs[3]; } palz_triangle;
@@ -306,6 +307,18 @@ typedef struct {
      * palz_lbvh_widen, used by the batched solver's occlusion rays. */
     const palz_wide_node* wide;
 } palz_scene_desc;
+
+/* Spectral resolve (pk.h:960-1031, main_gpu.cu:9-14, :473-483, :499-513):
+ * for each receiver point p, acc = (sum_l (R_p / pi) * xyz2srgblinear(cmf(l)
+ * / INTEGRAL_OF_CMF_Y_IN_NM) * d_images[l * n_points + p] * d_weights[l],
+ * n_frames) with the CIE 2015 10-degree fit, and pixel = packRGBA(srgb_oetf(
+ * acc.xyz / acc.w)).  d_images are the solver's per-wavelength irradiance
+ * images (palz_scene_desc.image), d_weights the sample weight of each
+ * wavelength (1 / p(lambda) for sampled wavelengths, a quadrature weight for
+ * a fixed set).  Either output may be NULL. */
+int palz_spectral_resolve(const float* d_images, const float* d_lambdas, const float* d_weights,
+                          const palz_first_diffuse* d_points, uint32_t n_points, uint32_t n_lambdas,
+                          float n_frames, palz_float4* d_accum, uint32_t* d_pixels, void* stream);
 
 /* Per (path, wavelength) outcome, item index = path * n_lambdas + l. */
 enum { PALZ_PATH_NOT_CONVERGED = 0, PALZ_PATH_CONVERGED = 1, PALZ_PATH_CONTRIBUTES = 2, PALZ_PATH_CULLED = 3 };

@@ -331,6 +331,20 @@ class BatchedSolver:
         return [dict(zip(keys, (int(v) for v in rep[6 * k:6 * k + 6]))) for k in range(3)]
 
 
+def spectral_resolve(d_images: DeviceBuffer | int, d_lambdas: DeviceBuffer | int, d_weights: DeviceBuffer | int,
+                     d_points: DeviceBuffer | int, n_points: int, n_lambdas: int, n_frames: float = 1.0,
+                     stream=None) -> tuple[DeviceBuffer, DeviceBuffer]:
+    """palz_spectral_resolve: (accumulators float4 x n_points, RGBA8 pixels) of
+    the per-wavelength irradiance images (pk.h:960-1031, main_gpu.cu:499-513)."""
+    ptr = lambda b: b.ptr if isinstance(b, DeviceBuffer) else int(b)  # noqa: E731
+    acc = DeviceBuffer(16 * max(n_points, 1))
+    pix = DeviceBuffer(4 * max(n_points, 1))
+    call("palz_spectral_resolve", C.c_void_p(ptr(d_images)), C.c_void_p(ptr(d_lambdas)),
+         C.c_void_p(ptr(d_weights)), C.c_void_p(ptr(d_points)), C.c_uint32(n_points), C.c_uint32(n_lambdas),
+         C.c_float(n_frames), C.c_void_p(acc.ptr), C.c_void_p(pix.ptr), _sp(stream))
+    return acc, pix
+
+
 def upload_scene_descs(descs: list[SceneDesc], stream=None) -> DeviceBuffer:
     arr = (SceneDesc * len(descs))(*descs)
     raw = np.frombuffer(bytes(arr), dtype=np.uint8)

@@ -1,0 +1,113 @@
+// Spectral resolve of the receiver irradiance: the CIE 2015 10-degree colour
+// matching fit, XYZ -> linear sRGB, the sRGB transfer function and the RGBA8
+// pack (pk.h:960-1031, main_gpu.cu:9-14, :473-483, :499-513).
+//
+// The reference weights each converged chain's contribution by
+// (R / pi) * xyz2srgblinear(cmf(lambda) / INTEGRAL_OF_CMF_Y_IN_NM) / p(lambda)
+// as it adds it to the pixel's float4 accumulator.  That weight depends on
+// the wavelength and the receiver point only, so here it is applied once per
+// (point, wavelength) to the solver's per-wavelength irradiance images:
+// accumulator = (sum_l (R / pi) * srgb(l) * E_l[p] * w_l, n_frames), then
+// pixel = packRGBA(srgb_oetf(acc.xyz / acc.w)).  w_l is the caller's sample
+// weight of wavelength l (1 / p(lambda) for the reference's sampled
+// wavelengths, a quadrature weight for a fixed set).
+#include "common.hpp"
+
+#include <cmath>
+
+namespace palz {
+namespace {
+
+constexpr float kIntegralOfCmfY = 118.51810464018001f;  // INTEGRAL_OF_CMF_Y_IN_NM (pk.h:1033)
+constexpr int kMaxLambdas = 256;
+
+__device__ __forceinline__ float asymmetric_gaussian(float x, float mean, float sigma, float a) {
+    float denom = sigma + a * (x - mean);
+    if (denom < 1.0e-15f) denom = 1.0e-15f;
+    if (sigma * 2.0f < denom) denom = sigma * 2.0f;
+    const float k = (x - mean) / denom;
+    return expf(-k * k);
+}
+__device__ __forceinline__ float cmf_x(float x) {
+    const float a = 0.42f * asymmetric_gaussian(x, 445.5849609375f, 31.146467208862305f, 0.06435633450746536f);
+    const float b = 1.16f * asymmetric_gaussian(x, 594.5570068359375f, 48.602108001708984f, -0.04772702232003212f);
+    return (a + b - a * b * 42.559776306152344f) * 1.0008530432426956f;
+}
+__device__ __forceinline__ float cmf_y(float x) {
+    return 1.0f * asymmetric_gaussian(x, 556.8383178710938f, 66.54190826416016f, -0.026492968201637268f) *
+           1.004315086937574f;
+}
+__device__ __forceinline__ float cmf_z(float x) {
+    return 2.146832f * asymmetric_gaussian(x, 445.9251708984375f, 30.91781997680664f, 0.08379141241312027f) *
+           0.9975112815948937f;
+}
+__device__ __forceinline__ float srgb_oetf(float r) {
+    r = fmaxf(r, 0.0f);
+    if (r <= 0.003130f) return 12.92f * r;
+    return 1.055f * powf(r, 1.0f / 2.4f) - 0.055f;
+}
+__device__ __forceinline__ uint32_t pack_rgba(float r, float g, float b) {
+    auto q = [](float c) -> uint32_t {
+        const int i = (int)(c * 255.0f + 0.5f);  // make_int4: truncation toward zero
+        return (uint32_t)min(max(i, 0), 255);
+    };
+    return 0xFF000000u | (q(b) << 16) | (q(g) << 8) | q(r);
+}
+
+// One thread per receiver point; the per-wavelength linear-sRGB weights are
+// computed once per workgroup into LDS.
+__global__ void __launch_bounds__(256)
+    k_spectral_resolve(const float* __restrict__ images, const float* __restrict__ lambdas,
+                       const float* __restrict__ weights, const palz_first_diffuse* __restrict__ points,
+                       uint32_t n_points, uint32_t n_lambdas, float n_frames, float4* __restrict__ accum,
+                       uint32_t* __restrict__ pixels) {
+    __shared__ float s_w[kMaxLambdas][3];
+    for (uint32_t l = threadIdx.x; l < n_lambdas; l += blockDim.x) {
+        const float lam = lambdas[l];
+        const float x = cmf_x(lam) / kIntegralOfCmfY, y = cmf_y(lam) / kIntegralOfCmfY,
+                    z = cmf_z(lam) / kIntegralOfCmfY;
+        // xyz2srgblinear (pk.h:1014-1022): dot products in the reference's order
+        const float w = weights[l];
+        s_w[l][0] = (3.2406f * x + -1.5372f * y + -0.4986f * z) * w;
+        s_w[l][1] = (-0.9689f * x + 1.8758f * y + 0.0415f * z) * w;
+        s_w[l][2] = (0.0557f * x + -0.204f * y + 1.057f * z) * w;
+    }
+    __syncthreads();
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_points) return;
+    float r = 0.0f, g = 0.0f, b = 0.0f;
+    for (uint32_t l = 0; l < n_lambdas; ++l) {
+        const float e = images[(size_t)l * n_points + p];
+        r += s_w[l][0] * e;
+        g += s_w[l][1] * e;
+        b += s_w[l][2] * e;
+    }
+    const palz_first_diffuse q = points[p];
+    const float k = 1.0f / 3.14159265358979323846f;
+    r *= q.R.x * k;
+    g *= q.R.y * k;
+    b *= q.R.z * k;
+    if (accum) accum[p] = make_float4(r, g, b, n_frames);
+    if (pixels) pixels[p] = pack_rgba(srgb_oetf(r / n_frames), srgb_oetf(g / n_frames), srgb_oetf(b / n_frames));
+}
+
+}  // namespace
+}  // namespace palz
+
+extern "C" int palz_spectral_resolve(const float* d_images, const float* d_lambdas, const float* d_weights,
+                                     const palz_first_diffuse* d_points, uint32_t n_points, uint32_t n_lambdas,
+                                     float n_frames, palz_float4* d_accum, uint32_t* d_pixels, void* stream) {
+    using namespace palz;
+    if (!d_images || !d_lambdas || !d_weights || !d_points)
+        return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_spectral_resolve: NULL input");
+    if (!d_accum && !d_pixels) return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_spectral_resolve: no output");
+    if (n_lambdas < 1 || n_lambdas > (uint32_t)kMaxLambdas)
+        return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_spectral_resolve: need 1..256 wavelengths");
+    if (!(n_frames > 0.0f)) return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_spectral_resolve: n_frames must be > 0");
+    if (n_points == 0u) return PALZ_OK;
+    k_spectral_resolve<<<div_up(n_points, 256u), 256, 0, (hipStream_t)stream>>>(
+        d_images, d_lambdas, d_weights, d_points, n_points, n_lambdas, n_frames,
+        reinterpret_cast<float4*>(d_accum), d_pixels);
+    PALZ_LAUNCH_CHECK();
+    return PALZ_OK;
+}
