@@ -254,6 +254,26 @@ def bench_c2(args) -> dict:
     resolve = {"ms": round(ms_resolve, 4), "bytes": resolve_bytes,
                "GBps": round(resolve_bytes / (ms_resolve * 1e-3) / 1e9, 1)}
 
+    # camera side: solvePrimary over a 1024^2 image of the same scene (lens on)
+    cam = CA.Camera.lookat((0.2, 3.0, 2.2), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 0.9, 1024, 1024)
+    ppr = CA.PrimaryParams()
+    ppr.p_light = CA.Float3(*s.p_light)
+    ppr.light_intensity, ppr.radiance_clamp = 16.0, 12.0
+    ppr.cauchy_a, ppr.cauchy_b = s.cauchy_a, s.cauchy_b
+    ppr.width, ppr.height, ppr.albedo = 1024, 1024, CA.Float3(0.8, 0.8, 0.8)
+    ppr.use_lens, ppr.lens_distance, ppr.lens_thickness, ppr.lens_R = 1, 0.3, 0.05, 0.6
+    d_fd, d_acc = CA.solve_primary(g, cam, ppr, stream=st)
+    st.sync()
+    t0.record(st)
+    for it in range(5):
+        ppr.iteration = it
+        CA.solve_primary(g, cam, ppr, d_fd, d_acc, stream=st)
+    t1.record(st)
+    st.sync()
+    ms_primary = t0.elapsed_ms(t1) / 5
+    primary = {"ms": round(ms_primary, 4), "pixels": 1024 * 1024,
+               "Mrays_per_s": round(1024 * 1024 / (ms_primary * 1e-3) / 1e6, 1)}
+
     # the same solve as one hipGraph launch
     g.clear_image()
     bs.solve(d_scenes, 1, L, CA.REFINE_VERTICES, d_res, graph=True)
@@ -369,6 +389,7 @@ def bench_c2(args) -> dict:
                   "image_rel_rmse_batched_vs_plain": img_rmse},
         "interval_culling": culling,
         "spectral_resolve": resolve,
+        "solve_primary": primary,
         "vertex_error": sample["vertex"],
         "parity_sample": sample["parity"],
         "pipeline_ms": {k: round(v, 3) for k, v in pipeline_ms.items()},

@@ -308,6 +308,47 @@ typedef struct {
     const palz_wide_node* wide;
 } palz_scene_desc;
 
+/* ---- camera side ------------------------------------------------------- */
+
+/* RayGenerator (camera.h:4-41): origin and the image-plane half axes
+ * (RayGenerator::lookat fills them from eye, target, up and fovy). */
+typedef struct {
+    palz_float3 origin;
+    palz_float3 right;
+    palz_float3 up;
+} palz_camera;
+
+/* What solvePrimary (main_gpu.cu:213-371) takes besides the scene.  The lens
+ * is LensParams (pk.h:1035-1040) placed in front of the camera (solveLens,
+ * pk.h:1042-1073); use_lens = 0 shoots the pinhole rays unrefracted.  The
+ * floor texture is an RGBX8 image sampled at (p.x, p.z) (Texture8RGBX::
+ * samplePoint, pk.h:1081-1094); without one every diffuse hit has `albedo`. */
+typedef struct {
+    palz_float3 p_light;
+    float light_intensity;   /* 16 (pk.h:1686) */
+    float radiance_clamp;    /* 12 (pk.h:1688) */
+    float cauchy_a, cauchy_b;
+    int32_t iteration;
+    uint32_t width, height;
+    uint32_t use_lens;
+    float lens_distance, lens_thickness, lens_R;
+    palz_float3 albedo;
+    const uint8_t* texture;  /* device RGBX8, or NULL */
+    uint32_t texture_width, texture_height;
+} palz_primary_params;
+
+/* solvePrimary (main_gpu.cu:213-371): one jittered camera ray per pixel
+ * through the lens to its first diffuse hit (up to 32 specular bounces, a
+ * Fresnel choice at dielectrics), a wavelength drawn from the CIE y
+ * distribution (CIE_2015_10deg::cmf_y_sample / cmf_y_pdf, pk.h:981-1011),
+ * d_first_diffuse[pixel] = {p, ng facing the ray, R, lambda} (only R = 0 is
+ * written when no diffuse surface is hit), and d_accum[pixel] += (direct
+ * light in linear sRGB / p(lambda), 1).  The exponentials and logarithms are
+ * this library's own FP32 routines (the reference uses the hardware's fast
+ * ones). */
+int palz_solve_primary(const palz_scene_view* scene, const palz_camera* camera, const palz_primary_params* pp,
+                       palz_first_diffuse* d_first_diffuse, palz_float4* d_accum, void* stream);
+
 /* Spectral resolve (pk.h:960-1031, main_gpu.cu:9-14, :473-483, :499-513):
  * for each receiver point p, acc = (sum_l (R_p / pi) * xyz2srgblinear(cmf(l)
  * / INTEGRAL_OF_CMF_Y_IN_NM) * d_images[l * n_points + p] * d_weights[l],

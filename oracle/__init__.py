@@ -38,6 +38,11 @@ _SIGS = {
     "orc_iv_residual": (C.c_int, [_P, _P, _P, C.c_int, C.c_float, C.c_float, _P]),
     "orc_interval_cull": (None, [_P, _P, _P, C.c_uint32, _P, _P, C.c_float, C.c_float, _P]),
     "orc_max_stack_depth": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
+    "orc_exp_f32": (C.c_float, [C.c_float]),
+    "orc_log_f32": (C.c_float, [C.c_float]),
+    "orc_cmf_y_sample": (C.c_float, [C.c_float]),
+    "orc_cmf_y_pdf": (C.c_float, [C.c_float]),
+    "orc_solve_primary": (None, [_P, _P, _P, C.c_uint32, _P, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -130,6 +135,23 @@ def normal_bounds(nodes: np.ndarray, attribs: np.ndarray) -> np.ndarray:
     out = np.zeros(max(n_int, 1), dtype=T.NORMAL_BOUND)
     lib().orc_normal_bounds(_ptr(np.ascontiguousarray(nodes)), n_int, _ptr(np.ascontiguousarray(attribs)), _ptr(out))
     return out[:n_int]
+
+
+def solve_primary(tris, attribs, nodes, root: int, camera, pp, texture=None):
+    """solvePrimary (main_gpu.cu:213-371) on the CPU: (FirstDiffuse records,
+    float4 accumulators) for a zeroed start.  `camera` / `pp` are the ctypes
+    structures of palzilla_amd.caustics; pp.texture is ignored (pass the RGBX8
+    image as `texture`)."""
+    import ctypes as _C
+    from palzilla_amd.scenes import FIRST_DIFFUSE
+    n = int(pp.width) * int(pp.height)
+    out = np.zeros(max(n, 1), dtype=FIRST_DIFFUSE)
+    acc = np.zeros((max(n, 1), 4), dtype=np.float32)
+    tex = None if texture is None else np.ascontiguousarray(texture, dtype=np.uint8)
+    lib().orc_solve_primary(_ptr(np.ascontiguousarray(tris, dtype=np.float32)), _ptr(np.ascontiguousarray(attribs)),
+                            _ptr(np.ascontiguousarray(nodes)), root, _C.byref(camera), _C.byref(pp), _ptr(tex),
+                            _ptr(out), _ptr(acc))
+    return out[:n], acc[:n]
 
 
 def validate_lbvh(nodes: np.ndarray, root: int, deltas: np.ndarray) -> bool:

@@ -967,4 +967,258 @@ int orc_max_stack_depth(const palz_triangle* tris, const palz_internal_node* nod
     return m;
 }
 
+// ---- solvePrimary (main_gpu.cu:213-371) and what it needs ----------------
+
+// This library's FP32 exp and log (the reference's NATIVE_EXPF / NATIVE_LOGF
+// are the hardware's fast forms): x = k ln2 + r with ln 2 in two parts, e^r
+// by its Taylor polynomial to degree 7, times 2^k in two factors, 0 below -87;
+// log m for m in [sqrt(1/2), sqrt(2)) as 2 atanh((m - 1) / (m + 1)) by the odd
+// series to s^9.
+static float pow2i(int e) {
+    uint32_t b = (uint32_t)(e + 127) << 23;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+float orc_exp_f32(float x) {
+    if (!(x == x)) return x;
+    if (x > 88.72283f) return INFINITY;
+    if (x < -87.0f) return 0.0f;
+    const float kf = std::floor(x * 1.44269504f + 0.5f);
+    const float r = (x - kf * 0.693145751953125f) - kf * 1.428606765330187e-06f;
+    float p = 1.98412698e-4f;
+    p = p * r + 1.38888889e-3f;
+    p = p * r + 8.33333333e-3f;
+    p = p * r + 4.16666667e-2f;
+    p = p * r + 1.66666667e-1f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    const int k = (int)kf, k1 = k / 2;
+    return p * pow2i(k1) * pow2i(k - k1);
+}
+float orc_log_f32(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -INFINITY : NAN;
+    if (x == INFINITY) return x;
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    int e = (int)(b >> 23) - 127;
+    if ((b >> 23) == 0u) {
+        const float y = x * 8388608.0f;
+        std::memcpy(&b, &y, 4);
+        e = (int)(b >> 23) - 127 - 23;
+    }
+    const uint32_t mb = (b & 0x7FFFFFu) | 0x3F800000u;
+    float m;
+    std::memcpy(&m, &mb, 4);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const float sv = (m - 1.0f) / (m + 1.0f);
+    const float s2 = sv * sv;
+    float q = 1.11111111e-1f;
+    q = q * s2 + 1.42857143e-1f;
+    q = q * s2 + 2.0e-1f;
+    q = q * s2 + 3.33333333e-1f;
+    q = q * s2 + 1.0f;
+    const float ef = (float)e;
+    return ef * 0.693145751953125f + (2.0f * sv * q + ef * 1.428606765330187e-06f);
+}
+
+namespace {
+float asym_gauss(float x, float mean, float sigma, float a) {  // pk.h:962-968
+    float denom = sigma + a * (x - mean);
+    if (denom < 1.0e-15f) denom = 1.0e-15f;
+    if (sigma * 2.0f < denom) denom = sigma * 2.0f;
+    float k = (x - mean) / denom;
+    return orc_exp_f32(-k * k);
+}
+float cmfX(float x) {  // pk.h:969-973
+    float a = 0.42f * asym_gauss(x, 445.5849609375f, 31.146467208862305f, 0.06435633450746536f);
+    float b = 1.16f * asym_gauss(x, 594.5570068359375f, 48.602108001708984f, -0.04772702232003212f);
+    return (a + b - a * b * 42.559776306152344f) * 1.0008530432426956f;
+}
+float cmfY(float x) {  // pk.h:974-976
+    return 1.0f * asym_gauss(x, 556.8383178710938f, 66.54190826416016f, -0.026492968201637268f) * 1.004315086937574f;
+}
+float cmfZ(float x) {  // pk.h:977-979
+    return 2.146832f * asym_gauss(x, 445.9251708984375f, 30.91781997680664f, 0.08379141241312027f) *
+           0.9975112815948937f;
+}
+float logisticPdf(float x, float sv) {  // pk.h:981-985 (double denominator, as the 1.0 literals make it)
+    float k = orc_exp_f32(-std::fabs(x) / sv);
+    return (float)(k / ((1.0 + k) * (1.0 + k) * sv));
+}
+float logisticCdf(float x, float sv) { return 1.0f / (1.0f + orc_exp_f32(-x / sv)); }  // pk.h:986-989
+float inverseLogisticCdf(float u, float sv) {                                          // pk.h:990-995
+    if (0.99999994f < u) u = 0.99999994f;
+    if (u < 1.175494351e-38f) u = 1.175494351e-38f;
+    return -sv * orc_log_f32(1.0f / u - 1.0f);
+}
+float cmfYPdf(float x) {  // pk.h:996-1002
+    float sx = x - 554.270751953125f;
+    float sv = 26.879621505737305f, a = -164.270751953125f, b = 275.729248046875f;
+    return logisticPdf(sx, sv) / (logisticCdf(b, sv) - logisticCdf(a, sv));
+}
+float cmfYSample(float u) {  // pk.h:1004-1010
+    float sv = 26.879621505737305f, a = -164.270751953125f, b = 275.729248046875f;
+    float Pa = logisticCdf(a, sv), Pb = logisticCdf(b, sv);
+    return inverseLogisticCdf(Pa + (Pb - Pa) * u, sv) + 554.270751953125f;
+}
+float3 xyz2srgblinear(float3 c) {  // pk.h:1014-1022
+    return {3.2406f * c.x + -1.5372f * c.y + -0.4986f * c.z, -0.9689f * c.x + 1.8758f * c.y + 0.0415f * c.z,
+            0.0557f * c.x + -0.204f * c.y + 1.057f * c.z};
+}
+float sphIntersectNear(float3 ro, float3 rd, float3 ce, float ra) {  // pk.h:52-65 (.x)
+    float len2rd = dot(rd, rd);
+    float3 oc = ro - ce;
+    float b = dot(oc, rd) / len2rd;
+    float c = dot(oc, oc) - ra * ra;
+    float h = b * b - c / len2rd;
+    if (h < 0.0f) return -1.0f;
+    h = std::sqrt(h);
+    return -b - h;
+}
+void solveLens(float3* ro, float3* rd, float3 camOrigin, float3 camForward, float distance, float thickness,
+               float R, float eta) {  // pk.h:1042-1073
+    float3 lensOrigin = camOrigin + camForward * distance;
+    float3 center1 = lensOrigin + camForward * (R - thickness * 0.5f);
+    float3 center2 = lensOrigin - camForward * (R - thickness * 0.5f);
+    float t1 = sphIntersectNear(*ro, *rd, center1, R);
+    if (t1 < 0.0f) return;
+    float3 hit1 = *ro + *rd * t1;
+    float3 wo;
+    refraction_norm_free(&wo, -*rd, hit1 - center1, eta);
+    *rd = wo;
+    *ro = hit1;
+    float t2 = sphIntersectNear(*ro, *rd, center2, R);
+    float3 hit2 = *ro + *rd * t2;
+    if (!refraction_norm_free(&wo, -*rd, hit2 - center2, eta)) return;
+    *rd = wo;
+    *ro = hit2;
+}
+float3 samplePoint(const uint8_t* tex, uint32_t w, uint32_t h, float u, float v) {  // pk.h:1081-1094
+    u = u - std::floor(u);
+    v = v - std::floor(v);
+    float fx = u * (float)w, fy = v * (float)h;
+    int ix = (int)(fx < (float)w - 1.0f ? fx : (float)w - 1.0f);
+    int iy = (int)(fy < (float)h - 1.0f ? fy : (float)h - 1.0f);
+    uint32_t idx = (uint32_t)ix + w * (uint32_t)iy;
+    float c[3];
+    for (int k = 0; k < 3; k++) {
+        float x = (float)tex[idx * 4u + k] / 255.0f;
+        c[k] = x == 0.0f ? 0.0f : orc_exp_f32(2.2f * orc_log_f32(x));  // powf(x, 2.2)
+    }
+    return {c[0], c[1], c[2]};
+}
+}  // namespace
+
+// solvePrimary (main_gpu.cu:213-371) for every pixel, serially; host
+// pointers, host root, host texture (or NULL).
+void orc_solve_primary(const palz_triangle* tris, const palz_triangle_attrib* attribs, const palz_internal_node* nodes,
+                       uint32_t root, const palz_camera* cam, const palz_primary_params* pp, const uint8_t* texture,
+                       palz_first_diffuse* out, float* accum) {
+    Scene sc{tris, attribs, nodes, root};
+    const float3 origin = load3(cam->origin), right = load3(cam->right), up = load3(cam->up);
+    const float3 forward = normalize(cross(up, right));
+    for (uint32_t yi = 0; yi < pp->height; yi++) {
+        for (uint32_t xi = 0; xi < pp->width; xi++) {
+            const uint32_t pixel = xi + yi * pp->width;
+            const uint32_t it = (uint32_t)pp->iteration;
+            uint32_t dimLevel = 0;
+            float jx, jy;
+            shuffled_scrambled_sobol_2d(&jx, &jy, it, xi, yi, dimLevel++);
+            float u = ((float)xi + jx) / (float)pp->width, v = ((float)yi + jy) / (float)pp->height;
+            float3 to = origin + forward + (-right + (right - -right) * u) + (up + (-up - up) * v);
+            float3 ro = origin, rd = to - origin;
+            float lx, ly;
+            shuffled_scrambled_sobol_2d(&lx, &ly, it, xi, yi, dimLevel++);
+            float lambda = cmfYSample(lx);
+            float p_lambda = cmfYPdf(lambda);
+            float eta = pp->cauchy_a + pp->cauchy_b / (lambda * lambda);
+            if (pp->use_lens) solveLens(&ro, &rd, origin, forward, pp->lens_distance, pp->lens_thickness, pp->lens_R, eta);
+            bool hasDiffuseHit = false;
+            Hit hit_last;
+            int max_sp = 0;
+            for (int d = 0; d < 32; d++) {
+                Hit hit;
+                intersect_stackfull(&hit, sc, ro, rd, invRd(rd), &max_sp);
+                if (hit.t == 3.402823466e+38F) {
+                    hit_last = hit;
+                    break;
+                }
+                const palz_triangle_attrib& attrib = attribs[hit.tri];
+                float rx, ry;
+                shuffled_scrambled_sobol_2d(&rx, &ry, it, xi, yi, dimLevel++);
+                if (attrib.material == PALZ_MATERIAL_DIFFUSE) {
+                    hasDiffuseHit = true;
+                    hit_last = hit;
+                    break;
+                }
+                float3 p_hit = ro + rd * hit.t;
+                float3 a0 = load3(attrib.shading_normals[0]);
+                float3 ns = a0 + (load3(attrib.shading_normals[1]) - a0) * hit.u +
+                            (load3(attrib.shading_normals[2]) - a0) * hit.v;
+                float3 wi = -rd;
+                bool eT = false;
+                if (attrib.material == PALZ_MATERIAL_DIELECTRIC) eT = !(rx < fresnel_exact_norm_free(wi, ns, eta));
+                float sign = 1.0f;
+                float3 wo;
+                if (!eT) {
+                    wo = reflection(wi, ns);
+                } else {
+                    if (!refraction_norm_free(&wo, wi, ns, eta)) break;
+                    sign = -1.0f;
+                }
+                if (0.0f < dot(hit.ng, wi) * dot(hit.ng, wo) * sign) {
+                    float3 offset_dir = normalize(hit.ng * dot(hit.ng, wo));
+                    ro = p_hit + offset_dir * rayOffsetScale(p_hit);
+                    rd = wo;
+                    continue;
+                }
+                break;
+            }
+            float* acc = accum ? accum + 4 * pixel : nullptr;
+            if (!hasDiffuseHit) {
+                out[pixel].R = {0.0f, 0.0f, 0.0f};
+                if (acc) acc[3] += 1.0f;
+                continue;
+            }
+            float3 p = ro + rd * hit_last.t;
+            float3 n = normalize(hit_last.ng);
+            if (0.0f < dot(n, rd)) n = -n;
+            float3 p_light = load3(pp->p_light);
+            float3 toLight = p_light - p;
+            float d2 = dot(toLight, toLight);
+            float3 refl = texture ? samplePoint(texture, pp->texture_width, pp->texture_height, p.x, p.z)
+                                  : load3(pp->albedo);
+            float3 L = {0.0f, 0.0f, 0.0f};
+            if (!occluded(sc, p, n, p_light, {0.0f, 0.0f, 0.0f})) {
+                float cosv = dot(normalize(toLight), n);
+                float contrib = pp->light_intensity / d2 * (cosv > 0.0f ? cosv : 0.0f);
+                contrib = contrib < pp->radiance_clamp ? contrib : pp->radiance_clamp;
+                float3 srgb = xyz2srgblinear(
+                    {cmfX(lambda) / 118.51810464018001f, cmfY(lambda) / 118.51810464018001f,
+                     cmfZ(lambda) / 118.51810464018001f});
+                float3 brdf = refl / 3.14159265358979323846f;
+                L = float3{brdf.x * srgb.x, brdf.y * srgb.y, brdf.z * srgb.z} * contrib / p_lambda;
+            }
+            out[pixel].p = {p.x, p.y, p.z};
+            out[pixel].ng = {n.x, n.y, n.z};
+            out[pixel].R = {refl.x, refl.y, refl.z};
+            out[pixel].lambda = lambda;
+            if (acc) {
+                acc[0] += L.x;
+                acc[1] += L.y;
+                acc[2] += L.z;
+                acc[3] += 1.0f;
+            }
+        }
+    }
+}
+
+float orc_cmf_y_sample(float u) { return cmfYSample(u); }
+float orc_cmf_y_pdf(float x) { return cmfYPdf(x); }
+
 }  // extern "C"

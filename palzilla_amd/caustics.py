@@ -33,6 +33,36 @@ class PhotonParams(C.Structure):
                 ("min_throughput", C.c_float), ("photons_per_triangle", C.c_uint32)]
 
 
+class Camera(C.Structure):
+    """palz_camera: RayGenerator's origin and image-plane half axes (camera.h:4-41)."""
+    _fields_ = [("origin", Float3), ("right", Float3), ("up", Float3)]
+
+    @staticmethod
+    def lookat(eye, center, up, fovy: float, width: int, height: int) -> "Camera":
+        """RayGenerator::lookat (camera.h:10-22), in FP32."""
+        f32 = np.float32
+        eye, center, up = (np.asarray(v, dtype=f32) for v in (eye, center, up))
+
+        def normalize(v):
+            return (v * (f32(1.0) / np.sqrt(f32(np.dot(v, v))))).astype(f32)
+
+        f = normalize(center - eye)
+        s = normalize(np.cross(f, up).astype(f32))
+        u = np.cross(s, f).astype(f32)
+        tan_y = f32(np.tan(f32(fovy) * f32(0.5)))
+        tan_x = f32(tan_y / f32(height) * f32(width))
+        return Camera(Float3(*eye), Float3(*(s * tan_x)), Float3(*(u * tan_y)))
+
+
+class PrimaryParams(C.Structure):
+    """palz_primary_params (what solvePrimary takes besides the scene)."""
+    _fields_ = [("p_light", Float3), ("light_intensity", C.c_float), ("radiance_clamp", C.c_float),
+                ("cauchy_a", C.c_float), ("cauchy_b", C.c_float), ("iteration", C.c_int32), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("use_lens", C.c_uint32), ("lens_distance", C.c_float),
+                ("lens_thickness", C.c_float), ("lens_R", C.c_float), ("albedo", Float3), ("texture", C.c_void_p),
+                ("texture_width", C.c_uint32), ("texture_height", C.c_uint32)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("tris", C.c_void_p), ("attribs", C.c_void_p), ("nodes", C.c_void_p), ("root", C.c_void_p),
                 ("points", C.c_void_p), ("lambdas", C.c_void_p), ("image", C.c_void_p),
@@ -329,6 +359,23 @@ class BatchedSolver:
              C.c_uint32(n_lambdas), int(refine), rep, _sp(self.stream))
         keys = ("block", "blocks_per_cu", "chunk", "finish_block", "newton_us", "finish_us")
         return [dict(zip(keys, (int(v) for v in rep[6 * k:6 * k + 6]))) for k in range(3)]
+
+
+def solve_primary(g: "GpuScene", camera: Camera, pp: PrimaryParams, d_out: DeviceBuffer | None = None,
+                  d_accum: DeviceBuffer | None = None, stream=None) -> tuple[DeviceBuffer, DeviceBuffer]:
+    """palz_solve_primary over the scene `g`: (FirstDiffuse per pixel,
+    float4 accumulators).  Fresh buffers start at zero."""
+    n = int(pp.width) * int(pp.height)
+    if d_out is None:
+        d_out = DeviceBuffer(max(n, 1) * FIRST_DIFFUSE.itemsize)
+        d_out.zero(stream)
+    if d_accum is None:
+        d_accum = DeviceBuffer(max(n, 1) * 16)
+        d_accum.zero(stream)
+    v = g.view()
+    call("palz_solve_primary", C.byref(v), C.byref(camera), C.byref(pp), C.c_void_p(d_out.ptr),
+         C.c_void_p(d_accum.ptr), _sp(stream))
+    return d_out, d_accum
 
 
 def spectral_resolve(d_images: DeviceBuffer | int, d_lambdas: DeviceBuffer | int, d_weights: DeviceBuffer | int,

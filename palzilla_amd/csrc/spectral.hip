@@ -11,36 +11,15 @@
 // pixel = packRGBA(srgb_oetf(acc.xyz / acc.w)).  w_l is the caller's sample
 // weight of wavelength l (1 / p(lambda) for the reference's sampled
 // wavelengths, a quadrature weight for a fixed set).
-#include "common.hpp"
+#include "spectral.hpp"
 
 #include <cmath>
 
 namespace palz {
 namespace {
 
-constexpr float kIntegralOfCmfY = 118.51810464018001f;  // INTEGRAL_OF_CMF_Y_IN_NM (pk.h:1033)
 constexpr int kMaxLambdas = 256;
 
-__device__ __forceinline__ float asymmetric_gaussian(float x, float mean, float sigma, float a) {
-    float denom = sigma + a * (x - mean);
-    if (denom < 1.0e-15f) denom = 1.0e-15f;
-    if (sigma * 2.0f < denom) denom = sigma * 2.0f;
-    const float k = (x - mean) / denom;
-    return expf(-k * k);
-}
-__device__ __forceinline__ float cmf_x(float x) {
-    const float a = 0.42f * asymmetric_gaussian(x, 445.5849609375f, 31.146467208862305f, 0.06435633450746536f);
-    const float b = 1.16f * asymmetric_gaussian(x, 594.5570068359375f, 48.602108001708984f, -0.04772702232003212f);
-    return (a + b - a * b * 42.559776306152344f) * 1.0008530432426956f;
-}
-__device__ __forceinline__ float cmf_y(float x) {
-    return 1.0f * asymmetric_gaussian(x, 556.8383178710938f, 66.54190826416016f, -0.026492968201637268f) *
-           1.004315086937574f;
-}
-__device__ __forceinline__ float cmf_z(float x) {
-    return 2.146832f * asymmetric_gaussian(x, 445.9251708984375f, 30.91781997680664f, 0.08379141241312027f) *
-           0.9975112815948937f;
-}
 __device__ __forceinline__ float srgb_oetf(float r) {
     r = fmaxf(r, 0.0f);
     if (r <= 0.003130f) return 12.92f * r;
@@ -66,11 +45,11 @@ __global__ void __launch_bounds__(256)
         const float lam = lambdas[l];
         const float x = cmf_x(lam) / kIntegralOfCmfY, y = cmf_y(lam) / kIntegralOfCmfY,
                     z = cmf_z(lam) / kIntegralOfCmfY;
-        // xyz2srgblinear (pk.h:1014-1022): dot products in the reference's order
         const float w = weights[l];
-        s_w[l][0] = (3.2406f * x + -1.5372f * y + -0.4986f * z) * w;
-        s_w[l][1] = (-0.9689f * x + 1.8758f * y + 0.0415f * z) * w;
-        s_w[l][2] = (0.0557f * x + -0.204f * y + 1.057f * z) * w;
+        const v3 c = xyz_to_srgb_linear(mk(x, y, z));
+        s_w[l][0] = c.x * w;
+        s_w[l][1] = c.y * w;
+        s_w[l][2] = c.z * w;
     }
     __syncthreads();
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
