@@ -393,6 +393,32 @@ int palz_solve_specular(int K, uint32_t event, const palz_scene_desc* scene, con
 int palz_interval_cull(const palz_scene_desc* d_scenes, uint32_t n_scenes, const palz_path* d_paths,
                        uint32_t n_paths, const uint32_t* d_count, uint8_t* d_admissible, void* stream);
 
+/* Node-level path cuts: traverseAdmissibleNodes<K> (pk.h:1760-2212), called
+ * by the reference from host code (main.cpp:930-944, :976-1070), here one
+ * walk per receiver point on the device.  For every point of `scene` with a
+ * diffuse hit, the K-tuples (K = event >> 24, 1..3) of triangles between
+ * scene->p_light and the point that survive the cut traversal over the scene's
+ * BVH and normal bounds (palz_lbvh_normal_bounds of the same build), with eta
+ * spanning [eta_lo, eta_hi].  The per-tuple test is the residual enclosure of
+ * palz_interval_cull, so no tuple with a solution the solvers accept is cut.
+ * Output: palz_path records grouped by point in point order, each point's
+ * tuples in the reference's traversal order (LIFO, child 1 first), with
+ * photon = 0xFFFFFFFF, `scene_index` as scene, and 1/3 as every start
+ * barycentric (pk.h:470-473).  *d_count (device) receives the total; records
+ * past `capacity` are dropped (capacity 0: count only).  A walk deeper than
+ * its stack (192 entries) stops early and sets *d_status (device, may be
+ * NULL) to PALZ_ERR_TRAVERSAL_STACK; a total of 2^32 tuples or more sets
+ * PALZ_ERR_CAPACITY, writes 0xFFFFFFFF as the count and no records.  At most
+ * 2^24 points per call.  The call synchronises `stream` once per level of its
+ * breadth-first expansion.  `scene` is a host struct holding device pointers;
+ * its points, light, triangles and BVH are read. */
+typedef struct palz_path_cuts palz_path_cuts;
+int palz_path_cuts_create(palz_path_cuts** out);
+int palz_path_cuts_destroy(palz_path_cuts* pc);
+int palz_path_cuts_enumerate(palz_path_cuts* pc, const palz_scene_desc* scene, const palz_normal_bound* d_normal_bounds,
+                             uint32_t event, float eta_lo, float eta_hi, uint32_t scene_index, palz_path* d_paths,
+                             uint32_t capacity, uint32_t* d_count, uint32_t* d_status, void* stream);
+
 /* ---- adjoint of receiver irradiance (no reference counterpart) -----------
  * Heights of a heightfield as parameters: a triangle's vertex c has height
  * parameter tri_vertex[3 t + c] (-1: the triangle is not part of the

@@ -37,6 +37,9 @@ _SIGS = {
     "orc_iv_op": (None, [C.c_int, C.c_float, C.c_float, _P]),
     "orc_iv_residual": (C.c_int, [_P, _P, _P, C.c_int, C.c_float, C.c_float, _P]),
     "orc_interval_cull": (None, [_P, _P, _P, C.c_uint32, _P, _P, C.c_float, C.c_float, _P]),
+    "orc_tuple_boxes_ok": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_uint32, C.c_float, C.c_float]),
+    "orc_path_cuts": (C.c_uint64, [_P, _P, _P, C.c_uint32, _P, _P, C.c_uint32, _P, C.c_uint32, C.c_float, C.c_float,
+                                   _P, _P, C.c_uint64]),
     "orc_max_stack_depth": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
     "orc_exp_f32": (C.c_float, [C.c_float]),
     "orc_log_f32": (C.c_float, [C.c_float]),
@@ -293,3 +296,30 @@ def interval_cull(scene, paths: np.ndarray, eta_lo: float, eta_hi: float) -> np.
     lib().orc_interval_cull(_ptr(scene.tris), _ptr(scene.attribs), _ptr(paths), paths.shape[0], _ptr(pl),
                             _ptr(scene.points), float(eta_lo), float(eta_hi), _ptr(out))
     return out
+
+
+def tuple_boxes_ok(vert_boxes, normal_boxes, p_beg, p_end, event: int, eta_lo: float, eta_hi: float) -> bool:
+    """Path-cuts test of one tuple of boxes, each (lo xyz, hi xyz)."""
+    vb = np.ascontiguousarray(np.asarray(vert_boxes, dtype=np.float32).reshape(-1, 6))
+    nb = np.ascontiguousarray(np.asarray(normal_boxes, dtype=np.float32).reshape(-1, 6))
+    pb = np.asarray(p_beg, dtype=np.float32)
+    pe = np.asarray(p_end, dtype=np.float32)
+    return bool(lib().orc_tuple_boxes_ok(_ptr(vb), _ptr(nb), vb.shape[0], _ptr(pb), _ptr(pe), int(event),
+                                         float(eta_lo), float(eta_hi)))
+
+
+def path_cuts(tris, attribs, nodes, root: int, nbounds, points, p_light, event: int, eta_lo: float, eta_hi: float):
+    """traverseAdmissibleNodes (pk.h:1760-2212) per receiver point:
+    (counts per point, tuples (total, K) in point order, each point's tuples in
+    traversal order)."""
+    K = int(event) >> 24
+    n = points.shape[0]
+    counts = np.zeros(max(n, 1), dtype=np.uint32)
+    pl = np.asarray(p_light, dtype=np.float32)
+    args = [_ptr(np.ascontiguousarray(tris)), _ptr(np.ascontiguousarray(attribs)), _ptr(np.ascontiguousarray(nodes)),
+            int(root), _ptr(np.ascontiguousarray(nbounds)), _ptr(np.ascontiguousarray(points)), n, _ptr(pl),
+            int(event), float(eta_lo), float(eta_hi), _ptr(counts)]
+    total = int(lib().orc_path_cuts(*args, None, 0))
+    tuples = np.zeros((max(total, 1), K), dtype=np.int32)
+    lib().orc_path_cuts(*args, _ptr(tuples), total)
+    return counts[:n], tuples[:total]

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 #include "oracle.hpp"
 
@@ -239,6 +240,111 @@ bool admissible(const palz_triangle* tris, const palz_triangle_attrib* attribs, 
     return false;
 }
 
+// ---- node-level path cuts (traverseAdmissibleNodes, pk.h:1760-2212) ----
+
+Iv3 aabb_iv(const palz_aabb& b) { return {{{b.lower.x, b.upper.x}, {b.lower.y, b.upper.y}, {b.lower.z, b.upper.z}}}; }
+
+Iv3 union_iv(const palz_aabb& a, const palz_aabb& b) {
+    return {{{box_min(a.lower.x, b.lower.x), box_max(a.upper.x, b.upper.x)},
+             {box_min(a.lower.y, b.lower.y), box_max(a.upper.y, b.upper.y)},
+             {box_min(a.lower.z, b.lower.z), box_max(a.upper.z, b.upper.z)}}};
+}
+
+// AABB::surfaceArea (minimum_lbvh.h:256-260) of the union of a node's children
+float node_area(const palz_internal_node& n) {
+    const Iv3 u = union_iv(n.aabbs[0], n.aabbs[1]);
+    const float sx = u.c[0].u - u.c[0].l, sy = u.c[1].u - u.c[1].l, sz = u.c[2].u - u.c[2].l;
+    return (sx * sy + sy * sz + sz * sx) * 2.0f;
+}
+
+// pk.h:1850-1875: p not behind the triangle, its normal turned to shading normal 0
+bool front_side(const palz_triangle& t, const palz_triangle_attrib& a, const float* p) {
+    const float e1[3] = {t.vs[1].x - t.vs[0].x, t.vs[1].y - t.vs[0].y, t.vs[1].z - t.vs[0].z};
+    const float e2[3] = {t.vs[2].x - t.vs[0].x, t.vs[2].y - t.vs[0].y, t.vs[2].z - t.vs[0].z};
+    float ng[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const palz_float3& s = a.shading_normals[0];
+    const float d = ng[0] * s.x + ng[1] * s.y + ng[2] * s.z;
+    for (float& c : ng) c = c * d;
+    const float lhs = ng[0] * p[0] + ng[1] * p[1] + ng[2] * p[2];
+    const float rhs = ng[0] * t.vs[0].x + ng[1] * t.vs[0].y + ng[2] * t.vs[0].z;
+    return !(lhs < rhs);
+}
+
+bool boxes_ok(const Iv3* vb, const Iv3* nb, int K, const float* pb, const float* pe, uint32_t event, Iv eta) {
+    Iv3 v[5];
+    v[0] = {{Iv::point(pb[0]), Iv::point(pb[1]), Iv::point(pb[2])}};
+    v[K + 1] = {{Iv::point(pe[0]), Iv::point(pe[1]), Iv::point(pe[2])}};
+    for (int k = 0; k < K; ++k) v[k + 1] = vb[k];
+    for (int k = 0; k < K; ++k)
+        if (!vertex_ok(v[k] - v[k + 1], v[k + 2] - v[k + 1], nb[k], (event >> k) & 1u, eta)) return false;
+    return true;
+}
+
+struct CutScene {
+    const palz_triangle* tris;
+    const palz_triangle_attrib* attribs;
+    const palz_internal_node* nodes;
+    const palz_normal_bound* nb;
+};
+
+// One query: the tuples in the order the reference's callback receives them.
+template <class Emit>
+void cut_walk(const CutScene& sc, uint32_t root, int K, const float* pb, const float* pe, uint32_t event, Iv eta,
+              Emit emit) {
+    const uint32_t kLeaf = PALZ_NODE_LEAF_BIT;
+    struct Entry { uint32_t n[3]; };
+    std::vector<Entry> stack;
+    Entry cur = {{root, root, root}};
+    while (true) {
+        int cut = -1;
+        float best = -1.0f;
+        for (int k = 0; k < K; ++k) {
+            if (cur.n[k] & kLeaf) continue;
+            const float a = node_area(sc.nodes[cur.n[k]]);
+            if (best < a) {
+                best = a;
+                cut = k;
+            }
+        }
+        if (cut < 0) {
+            int idx[3];
+            for (int k = 0; k < K; ++k) idx[k] = (int)(cur.n[k] & ~kLeaf);
+            bool ok = true;
+            for (int k = 0; k + 1 < K; ++k)
+                if (idx[k] == idx[k + 1]) ok = false;
+            if (ok && !front_side(sc.tris[idx[0]], sc.attribs[idx[0]], pb)) ok = false;
+            if (ok && !front_side(sc.tris[idx[K - 1]], sc.attribs[idx[K - 1]], pe)) ok = false;
+            if (ok) emit(idx);
+        } else {
+            Iv3 vb[3], nbx[3];
+            for (int k = 0; k < K; ++k) {
+                if (k == cut) continue;
+                const uint32_t n = cur.n[k];
+                if (n & kLeaf) {
+                    vb[k] = hull(sc.tris[n & ~kLeaf].vs);
+                    nbx[k] = hull(sc.attribs[n & ~kLeaf].shading_normals);
+                } else {
+                    vb[k] = union_iv(sc.nodes[n].aabbs[0], sc.nodes[n].aabbs[1]);
+                    nbx[k] = union_iv(sc.nb[n].normal_bounds[0], sc.nb[n].normal_bounds[1]);
+                }
+            }
+            const uint32_t c = cur.n[cut];
+            for (int i = 0; i < 2; ++i) {
+                vb[cut] = aabb_iv(sc.nodes[c].aabbs[i]);
+                nbx[cut] = aabb_iv(sc.nb[c].normal_bounds[i]);
+                if (boxes_ok(vb, nbx, K, pb, pe, event, eta)) {
+                    Entry e = cur;
+                    e.n[cut] = sc.nodes[c].children[i];
+                    stack.push_back(e);
+                }
+            }
+        }
+        if (stack.empty()) break;
+        cur = stack.back();
+        stack.pop_back();
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -288,6 +394,48 @@ void orc_interval_cull(const palz_triangle* tris, const palz_triangle_attrib* at
         const float pe[3] = {q.p.x, q.p.y, q.p.z};
         out[p] = admissible(tris, attribs, paths[p], p_light, pe, {eta_lo, eta_hi}) ? 1 : 0;
     }
+}
+
+// Path-cuts test of one tuple of boxes (K x 6 floats each: lo xyz, hi xyz):
+// 1 if every vertex of light -> boxes -> p_end passes.
+int orc_tuple_boxes_ok(const float* vb, const float* nb, int K, const float* p_beg, const float* p_end,
+                       uint32_t event, float eta_lo, float eta_hi) {
+    Iv3 v[3], n[3];
+    for (int k = 0; k < K; ++k) {
+        const float* b = vb + 6 * k;
+        const float* m = nb + 6 * k;
+        v[k] = {{{b[0], b[3]}, {b[1], b[4]}, {b[2], b[5]}}};
+        n[k] = {{{m[0], m[3]}, {m[1], m[4]}, {m[2], m[5]}}};
+    }
+    return boxes_ok(v, n, K, p_beg, p_end, event, {eta_lo, eta_hi}) ? 1 : 0;
+}
+
+// traverseAdmissibleNodes<K> (pk.h:1760-2212) for every receiver point with a
+// diffuse hit: counts[q] = its tuple count; the tuples (K ints each, points in
+// order, each point's tuples in traversal order) fill `tuples` up to
+// `capacity` tuples.  Returns the total.
+uint64_t orc_path_cuts(const palz_triangle* tris, const palz_triangle_attrib* attribs, const palz_internal_node* nodes,
+                       uint32_t root, const palz_normal_bound* nb, const palz_first_diffuse* points, uint32_t n_points,
+                       const float* p_light, uint32_t event, float eta_lo, float eta_hi, uint32_t* counts,
+                       int32_t* tuples, uint64_t capacity) {
+    const int K = (int)(event >> 24);
+    const CutScene sc = {tris, attribs, nodes, nb};
+    uint64_t total = 0;
+    for (uint32_t q = 0; q < n_points; ++q) {
+        const palz_first_diffuse& pt = points[q];
+        uint32_t found = 0;
+        if (!(pt.R.x == 0.0f && pt.R.y == 0.0f && pt.R.z == 0.0f)) {
+            const float pe[3] = {pt.p.x, pt.p.y, pt.p.z};
+            cut_walk(sc, root, K, p_light, pe, event, {eta_lo, eta_hi}, [&](const int* idx) {
+                if (total < capacity)
+                    for (int k = 0; k < K; ++k) tuples[total * K + k] = idx[k];
+                ++total;
+                ++found;
+            });
+        }
+        counts[q] = found;
+    }
+    return total;
 }
 
 }  // extern "C"

@@ -392,6 +392,54 @@ def spectral_resolve(d_images: DeviceBuffer | int, d_lambdas: DeviceBuffer | int
     return acc, pix
 
 
+class PathCuts:
+    """palz_path_cuts_*: node-level path cuts (traverseAdmissibleNodes,
+    pk.h:1760-2212) from the light to every receiver point of a scene,
+    emitting palz_path records that the solvers take as they are."""
+
+    def __init__(self, stream=None):
+        h = C.c_void_p()
+        call("palz_path_cuts_create", C.byref(h))
+        self.handle = h.value
+        self.stream = stream
+        self.d_count = DeviceBuffer(16)
+        self.d_status = DeviceBuffer(16)
+        self.d_status.zero(stream)
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            try:
+                lib().palz_path_cuts_destroy(C.c_void_p(self.handle))
+            except Exception:
+                pass
+            self.handle = None
+
+    def enumerate(self, desc: SceneDesc, d_normal_bounds: DeviceBuffer | int, event: int, eta_lo: float,
+                  eta_hi: float, capacity: int, d_paths: DeviceBuffer | None = None,
+                  scene_index: int = 0) -> tuple[DeviceBuffer | None, int]:
+        """(path buffer, total count); the buffer holds min(total, capacity) paths."""
+        if d_paths is None and capacity:
+            d_paths = DeviceBuffer(capacity * PATH.itemsize)
+        nb = d_normal_bounds.ptr if isinstance(d_normal_bounds, DeviceBuffer) else int(d_normal_bounds)
+        call("palz_path_cuts_enumerate", C.c_void_p(self.handle), C.byref(desc), C.c_void_p(nb), C.c_uint32(event),
+             C.c_float(eta_lo), C.c_float(eta_hi), C.c_uint32(scene_index),
+             C.c_void_p(d_paths.ptr if d_paths else None), C.c_uint32(capacity), C.c_void_p(self.d_count.ptr),
+             C.c_void_p(self.d_status.ptr), _sp(self.stream))
+        n = int(self.d_count.download(np.uint32, 1, self.stream)[0])
+        return d_paths, n
+
+    def status(self) -> int:
+        return int(self.d_status.download(np.uint32, 1, self.stream)[0])
+
+
+def eta_range(scene: CausticScene) -> tuple[float, float]:
+    """The refractive indices of the scene's wavelengths as the solvers
+    compute them (float32 Cauchy), as (lowest, highest)."""
+    lam = scene.lambdas.astype(np.float32)
+    eta = np.float32(scene.cauchy_a) + np.float32(scene.cauchy_b) / (lam * lam)
+    return float(eta.min()), float(eta.max())
+
+
 def upload_scene_descs(descs: list[SceneDesc], stream=None) -> DeviceBuffer:
     arr = (SceneDesc * len(descs))(*descs)
     raw = np.frombuffer(bytes(arr), dtype=np.uint8)
