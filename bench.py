@@ -533,6 +533,56 @@ def bench_c4(args) -> dict:
 # ---------------------------------------------------------------------------------------------------
 # adjoint: d(L2 image loss)/d(heights) of a 128^2-cell refractive heightfield
 
+def bench_path_cuts(args) -> dict:
+    """Node-level path cuts (palz_path_cuts_enumerate) of TT chains on a 16^2-cell
+    slab (516 triangles) to a 32^2 receiver, 16 wavelengths: count-only and
+    full (count + records) calls, timed with the stream synchronised."""
+    import time
+
+    from palzilla_amd import caustics as CA
+    from palzilla_amd import scenes
+    from palzilla_amd.device import DeviceBuffer, Stream
+
+    s = scenes.config2(cells=16, width=32, photons=1 << 14, n_lambdas=16)
+    st = Stream()
+    g = CA.GpuScene(s, stream=st)
+    d_nb = g.bvh.normal_bounds(g.d_attribs.ptr, st)
+    pc = CA.PathCuts(stream=st)
+    event, eta = CA.event_u32("TT"), CA.eta_range(s)
+    desc = g.desc()
+    _, n = pc.enumerate(desc, d_nb, event, *eta, capacity=0)
+    st.sync()
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        _, n = pc.enumerate(desc, d_nb, event, *eta, capacity=0)
+    st.sync()
+    ms_count = (time.perf_counter() - t) * 1e3 / reps
+    d_paths = DeviceBuffer(max(n, 1) * CA.PATH.itemsize)
+    pc.enumerate(desc, d_nb, event, *eta, capacity=n, d_paths=d_paths)
+    st.sync()
+    t = time.perf_counter()
+    for _ in range(reps):
+        pc.enumerate(desc, d_nb, event, *eta, capacity=n, d_paths=d_paths)
+    st.sync()
+    ms_full = (time.perf_counter() - t) * 1e3 / reps
+    assert pc.status() == 0
+    out = {"workload": "TT path cuts, 16^2-cell slab (516 tris), 32^2 receiver points, eta over 16 wavelengths",
+           "tuples": n, "count_only_ms": round(ms_count, 2), "full_ms": round(ms_full, 2),
+           "tuples_per_s": round(n / (ms_full * 1e-3), 1), "record_bytes": n * CA.PATH.itemsize}
+    if not args.no_cpu:
+        import oracle
+        nodes, root = g.bvh.download_nodes(), g.bvh.download_root()
+        nb = oracle.normal_bounds(nodes, s.attribs)
+        pts = s.points[[0, 341, 682, 1023]]
+        t = time.perf_counter()
+        _, tup = oracle.path_cuts(s.tris, s.attribs, nodes, root, nb, pts, s.p_light, event, *eta)
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(tup.shape[0] / dt, 1), "unit": "tuples/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle traversal of 4 receiver points ({tup.shape[0]} tuples, {dt:.1f} s)"}
+    return out
+
+
 def bench_c5(args) -> dict:
     from palzilla_amd import caustics as CA
     from palzilla_amd import scenes
@@ -684,7 +734,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5", "lbvh"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5", "lbvh", "pathcuts"])
     ap.add_argument("--no-extras", action="store_true", help="c2 only, without the c4 and lbvh sub-results")
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--cpu-sample", type=int, default=1 << 22, help="LBVH CPU baseline triangles")
@@ -705,6 +755,12 @@ def main() -> int:
             out["c4"] = bench_c4(args)
             out["c5"] = bench_c5(args)
             out["lbvh"] = bench_lbvh(args)
+            out["path_cuts"] = bench_path_cuts(args)
+    elif args.workload == "pathcuts":
+        r = bench_path_cuts(args)
+        out = {"metric": METRIC, "value": r["tuples_per_s"], "unit": "admissible tuples/s", "ms_per_step": r["full_ms"],
+               "dtype": "f32 (directed-rounding intervals)", "config": {"workload": r["workload"]},
+               "cpu_baseline": r.get("cpu_baseline"), "path_cuts": r}
     elif args.workload == "c5":
         r = bench_c5(args)
         out = {"metric": METRIC, "value": round(r["params"] / (r["backward_ms"] * 1e-3), 1),

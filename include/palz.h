@@ -308,6 +308,23 @@ typedef struct {
     const palz_wide_node* wide;
 } palz_scene_desc;
 
+/* ---- scene ingestion --------------------------------------------------
+ * A Wavefront OBJ file as triangles and attributes, with loadFrame's rules
+ * (pk.h:1246-1362; its Alembic input, scene.abc, is not part of the
+ * reference): positions and shading normals per face corner, the material
+ * named by the `usemtl` in force ("mirror", "dielectric"; any other name or
+ * none: `default_material`, a PALZ_MATERIAL_* value; loadFrame's default is
+ * Diffuse).  Faces of n > 3 corners become the fan (0, i, i + 1) in file
+ * order (loadFrame asserts triangles, pk.h:1320); a corner without a normal
+ * takes the triangle's normalised geometric normal.  Indices may be negative
+ * (relative).  Host memory only: palz_mesh_copy fills caller buffers of
+ * palz_mesh_triangle_count entries (either may be NULL). */
+typedef struct palz_mesh palz_mesh;
+int palz_mesh_load_obj(const char* path, int default_material, palz_mesh** out);
+int palz_mesh_triangle_count(const palz_mesh* m, uint32_t* n);
+int palz_mesh_copy(const palz_mesh* m, palz_triangle* tris, palz_triangle_attrib* attribs);
+int palz_mesh_destroy(palz_mesh* m);
+
 /* ---- camera side ------------------------------------------------------- */
 
 /* RayGenerator (camera.h:4-41): origin and the image-plane half axes

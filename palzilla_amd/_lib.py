@@ -69,6 +69,10 @@ _SIGS = {
                                       C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "palz_solve_specular": (C.c_int, [C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                       C.c_void_p, C.c_void_p]),
+    "palz_mesh_load_obj": (C.c_int, [C.c_char_p, C.c_int, C.c_void_p]),
+    "palz_mesh_triangle_count": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "palz_mesh_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "palz_mesh_destroy": (C.c_int, [C.c_void_p]),
     "palz_path_cuts_create": (C.c_int, [C.c_void_p]),
     "palz_path_cuts_destroy": (C.c_int, [C.c_void_p]),
     "palz_path_cuts_enumerate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float,

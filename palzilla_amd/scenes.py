@@ -11,6 +11,7 @@ the pixel centres of a W x H grid on that plane.
 """
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -354,21 +355,50 @@ def triangle_soup(n: int, seed: int = 3, mu: float = -6.0, sigma: float = 0.5) -
     return v.reshape(n, 9).astype(np.float32)
 
 
+def load_obj_mesh(path: str, default_material: int = T.MATERIAL_DIFFUSE) -> tuple[np.ndarray, np.ndarray]:
+    """Triangles (n, 9) float32 and TriangleAttrib records of a Wavefront OBJ
+    file, read by the library's loader (palz_mesh_load_obj: loadFrame's rules,
+    pk.h:1246-1362, faces fan-triangulated in file order)."""
+    import ctypes as C
+
+    from ._lib import call, lib
+    h = C.c_void_p()
+    call("palz_mesh_load_obj", os.fsencode(path), int(default_material), C.byref(h))
+    try:
+        n = C.c_uint32()
+        call("palz_mesh_triangle_count", h, C.byref(n))
+        tris = np.zeros((n.value, 9), dtype=np.float32)
+        att = np.zeros(n.value, dtype=T.TRIANGLE_ATTRIB)
+        call("palz_mesh_copy", h, tris.ctypes.data_as(C.c_void_p), att.ctypes.data_as(C.c_void_p))
+    finally:
+        lib().palz_mesh_destroy(h)
+    return tris, att
+
+
+def obj_scene(path: str, *, lift: float = 1.0, photons: int = 1 << 16, width: int = 64, n_lambdas: int = 4,
+              cauchy_a: float = 1.5, cauchy_b: float = 4200.0) -> CausticScene:
+    """An OBJ mesh as a glass object over a receiver: every face dielectric
+    (unless the file names another material), normals from the file, lifted by
+    `lift` in y, light (0.3, 3, 0.2), diffuse floor quad at y = 0 over [-3, 3]^2, receiver [-1.25, 1.25]^2 at width^2, Cauchy dispersion on
+    n_lambdas wavelengths over [380, 720] nm."""
+    t, a = load_obj_mesh(path, T.MATERIAL_DIELECTRIC)
+    t = t.copy()
+    t[:, 1::3] += np.float32(lift)
+    tr, nr = quad(0.0, (-3.0, 3.0, -3.0, 3.0), 1.0)
+    ft, fa = _assemble([(tr, nr, T.MATERIAL_DIFFUSE)])
+    tris = np.concatenate([t, ft])
+    att = np.concatenate([a, fa])
+    lam = np.linspace(380.0, 720.0, n_lambdas).astype(np.float32)
+    ext = (-1.25, 1.25, -1.25, 1.25)
+    return CausticScene(tris=tris, attribs=att, p_light=(0.3, 3.0, 0.2), points=receiver_points(width, width, ext),
+                        width=width, height=width, extent=ext, cauchy_a=cauchy_a, cauchy_b=cauchy_b, lambdas=lam,
+                        photons_per_triangle=max(1, photons // t.shape[0]),
+                        eta_min=cauchy_a + cauchy_b / 380.0 ** 2, eta_max=cauchy_a + cauchy_b / 720.0 ** 2,
+                        name=f"{os.path.basename(path)} as glass, {t.shape[0]} tris, {width}^2, {photons} photons",
+                        info={"path": path})
+
+
 def load_obj(path: str) -> np.ndarray:
-    """Triangles of a Wavefront OBJ file, (n, 9) float32: `v` records, and
-    each `f` record fan-triangulated in file order (v0, vi, vi+1)."""
-    verts, tris = [], []
-    with open(path) as f:
-        for line in f:
-            parts = line.split()
-            if not parts:
-                continue
-            if parts[0] == "v":
-                verts.append([float(x) for x in parts[1:4]])
-            elif parts[0] == "f":
-                idx = [int(p.split("/")[0]) for p in parts[1:]]
-                idx = [i - 1 if i > 0 else len(verts) + i for i in idx]
-                for j in range(1, len(idx) - 1):
-                    tris.append(idx[0:1] + idx[j:j + 2])
-    v = np.asarray(verts, dtype=np.float32)
-    return v[np.asarray(tris, dtype=np.int64)].reshape(-1, 9)
+    """Triangles of a Wavefront OBJ file, (n, 9) float32, each face
+    fan-triangulated in file order (v0, vi, vi+1)."""
+    return load_obj_mesh(path)[0]
