@@ -42,6 +42,7 @@ constexpr uint32_t kCutsMaxThreads = 2048u * kCutsThreads;  // resident walkers
 constexpr uint32_t kCutsExpandTarget = 1u << 20;  // listed tuples: work for every walker
 constexpr uint32_t kCutsListCap = 1u << 24;       // receiver points per call
 constexpr int kCutsMaxLevels = 64;
+constexpr uint32_t kCutsOnePassCap = 1u << 26;   // records: larger capacities take the two-pass walk
 
 struct CutsArgs {
     const palz_triangle* tris;
@@ -62,6 +63,7 @@ struct CutsArgs {
     uint32_t capacity;
     uint32_t* status;
     unsigned long long* total;  // 64-bit sum of the counts; all ones once it does not fit 32 bits
+    uint64_t* words;            // one-pass mode: (tuple << 32 | record position) per record
 };
 
 // A pending tuple of the traversal: receiver point and one node per slot.
@@ -245,9 +247,15 @@ __global__ void __launch_bounds__(256) k_expand(CutsArgs a, const CutEntry* __re
 // The walk of every listed tuple's subtree on a LIFO stack in global memory,
 // persistent: a lane that finishes a tuple takes the next one from a queue
 // (one atomic per wave), so lanes of a wave do not wait for the longest walk.
-// Counting pass (per-tuple counts and their 64-bit sum), then, after the scan,
-// the writing pass at the tuples' offsets.
-template <int K, bool kWrite>
+// Modes: kCount (per-tuple counts and their 64-bit sum), kWrite (after the
+// scan: records at the tuples' offsets), kEmit (one pass: records in arrival
+// order at positions taken from a 64-bit counter, one atomic per wave, each
+// with the word tuple << 32 | position; a stable sort of the words by tuple
+// then restores the traversal order, since one lane emits a tuple's records
+// at increasing positions).
+enum WalkMode { kCount = 0, kWrite = 1, kEmit = 2 };
+
+template <int K, int kMode>
 __global__ void __launch_bounds__(kCutsThreads) k_walk(CutsArgs a, const CutEntry* __restrict__ entries,
                                                        uint32_t n_entries, uint32_t* __restrict__ queue) {
     // n_threads is a multiple of the block size and every lane stays in the
@@ -259,7 +267,7 @@ __global__ void __launch_bounds__(kCutsThreads) k_walk(CutsArgs a, const CutEntr
     const uint32_t stride = a.n_threads;  // words between consecutive entries of one thread
     const iv::I eta = {a.eta_lo, a.eta_hi};
     const v3 pb = mk(a.lx, a.ly, a.lz);
-    if (kWrite && *a.total == ~0ull) return;
+    if (kMode == kWrite && *a.total == ~0ull) return;
     unsigned long long sum = 0;
     constexpr uint32_t kNone = 0xFFFFFFFFu;
     uint32_t e = kNone;
@@ -290,9 +298,9 @@ __global__ void __launch_bounds__(kCutsThreads) k_walk(CutsArgs a, const CutEntr
                     for (int k = 0; k < K; ++k) cur[k] = en.n[k];
                     top = 0;
                     found = 0;
-                    base = kWrite ? a.counts[idx] : 0u;
+                    base = kMode == kWrite ? a.counts[idx] : 0u;
                     if (!has_hit(pt)) {
-                        if (!kWrite) a.counts[idx] = 0u;
+                        if (kMode == kCount) a.counts[idx] = 0u;
                         e = kNone;
                     }
                 } else {
@@ -307,9 +315,22 @@ __global__ void __launch_bounds__(kCutsThreads) k_walk(CutsArgs a, const CutEntr
         uint32_t child[2];
         bool finished = false;
         const int cut = cut_step<K>(a, cur, pb, pe, eta, ok, child);
+        const bool emit = cut < 0 && leaf_tuple_valid<K>(a, cur, pb, pe);
+        uint64_t pos = base + found;
+        if (kMode == kEmit) {
+            // positions for this step's records, one atomic per wave
+            const uint64_t em = __ballot(emit);
+            if (em) {
+                const int leader = __ffsll((unsigned long long)em) - 1;
+                unsigned long long first = 0;
+                if ((int)lane == leader) first = atomicAdd(a.total, (unsigned long long)__popcll(em));
+                first = __shfl(first, leader);
+                pos = first + (uint64_t)__popcll(em & ((1ull << lane) - 1ull));
+            }
+        }
         if (cut < 0) {
-            if (leaf_tuple_valid<K>(a, cur, pb, pe)) {
-                if (kWrite && base + found < a.capacity) {
+            if (emit) {
+                if (kMode != kCount && pos < a.capacity) {
                     palz_path p;
                     p.scene = a.scene;
                     p.pixel = q;
@@ -322,7 +343,8 @@ __global__ void __launch_bounds__(kCutsThreads) k_walk(CutsArgs a, const CutEntr
                         p.params[2 * k] = k < K ? 1.0f / 3.0f : 0.0f;
                         p.params[2 * k + 1] = k < K ? 1.0f / 3.0f : 0.0f;
                     }
-                    a.out[base + found] = p;
+                    a.out[pos] = p;
+                    if (kMode == kEmit) a.words[pos] = (uint64_t)e << 32 | pos;
                 }
                 ++found;
             }
@@ -345,12 +367,12 @@ __global__ void __launch_bounds__(kCutsThreads) k_walk(CutsArgs a, const CutEntr
 #pragma unroll
             for (int k = 0; k < K; ++k) cur[k] = stack[(top * K + k) * stride];
         } else {
-            if (!kWrite) a.counts[e] = found;
+            if (kMode == kCount) a.counts[e] = found;
             sum += found;
             e = kNone;
         }
     }
-    if (!kWrite) {
+    if (kMode == kCount) {
         // one 64-bit atomic per wave
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
@@ -383,14 +405,23 @@ void launch_expand(const CutsArgs& a, const CutEntry* in, uint32_t n_in, uint32_
 
 template <int K>
 void launch_walk(const CutsArgs& a, const CutEntry* entries, uint32_t n_entries, uint32_t* queue, hipStream_t st,
-                 bool write) {
+                 int mode) {
     const uint32_t blocks = div_up(a.n_threads, (uint32_t)kCutsThreads);
     k_write_zero<<<1, 1, 0, st>>>(queue);
-    if (write)
-        k_walk<K, true><<<blocks, kCutsThreads, 0, st>>>(a, entries, n_entries, queue);
-    else
-        k_walk<K, false><<<blocks, kCutsThreads, 0, st>>>(a, entries, n_entries, queue);
+    if (mode == kCount) k_walk<K, kCount><<<blocks, kCutsThreads, 0, st>>>(a, entries, n_entries, queue);
+    if (mode == kWrite) k_walk<K, kWrite><<<blocks, kCutsThreads, 0, st>>>(a, entries, n_entries, queue);
+    if (mode == kEmit) k_walk<K, kEmit><<<blocks, kCutsThreads, 0, st>>>(a, entries, n_entries, queue);
 }
+
+// records in sorted-word order: out[i] = scratch[position of word i]
+__global__ void __launch_bounds__(256) k_gather_records(const uint64_t* __restrict__ words,
+                                                        const palz_path* __restrict__ scratch, uint32_t n,
+                                                        palz_path* __restrict__ out) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        out[i] = scratch[(uint32_t)words[i]];
+}
+
+__global__ void k_write_count(uint32_t* count, const unsigned long long* total) { *count = (uint32_t)*total; }
 
 }  // namespace
 
@@ -405,6 +436,27 @@ struct palz_path_cuts {
     uint32_t* stack = nullptr;
     unsigned long long* total = nullptr;
     uint32_t* scalars = nullptr;  // [0] expandable flag, [1] level total, [2] walk queue
+    // one-pass mode: unsorted records, their words and the sort's tmp
+    uint32_t cap_emit = 0;
+    palz_path* emit_records = nullptr;
+    uint64_t* emit_words = nullptr;
+    uint64_t* emit_tmp = nullptr;
+    palz_sorter* sorter = nullptr;
+
+    int ensure_emit(uint32_t n) {
+        if (!sorter && palz_sorter_create(&sorter) != PALZ_OK) return PALZ_ERR_OUT_OF_MEMORY;
+        if (n <= cap_emit) return PALZ_OK;
+        for (void* p : {(void*)emit_records, (void*)emit_words, (void*)emit_tmp})
+            if (p) (void)hipFree(p);
+        emit_records = nullptr;
+        emit_words = emit_tmp = nullptr;
+        cap_emit = 0;
+        PALZ_HIP(hipMalloc(&emit_records, sizeof(palz_path) * n));
+        PALZ_HIP(hipMalloc(&emit_words, sizeof(uint64_t) * n));
+        PALZ_HIP(hipMalloc(&emit_tmp, sizeof(uint64_t) * n));
+        cap_emit = n;
+        return PALZ_OK;
+    }
 
     int ensure_lists(uint32_t n_entries) {
         if (n_entries > cap_entries) {
@@ -435,8 +487,9 @@ struct palz_path_cuts {
     }
     ~palz_path_cuts() {
         for (void* p : {(void*)list[0], (void*)list[1], (void*)counts, (void*)scan_sums, (void*)stack, (void*)total,
-                        (void*)scalars})
+                        (void*)scalars, (void*)emit_records, (void*)emit_words, (void*)emit_tmp})
             if (p) (void)hipFree(p);
+        if (sorter) palz_sorter_destroy(sorter);
     }
 };
 
@@ -496,6 +549,7 @@ int palz_path_cuts_enumerate(palz_path_cuts* pc, const palz_scene_desc* scene, c
     a.capacity = capacity;
     a.status = d_status;
     a.total = pc->total;
+    a.words = nullptr;
 
     // Breadth-first expansion until the list holds enough tuples to fill the
     // device: one host read of the level's total per level.
@@ -533,19 +587,52 @@ int palz_path_cuts_enumerate(palz_path_cuts* pc, const palz_scene_desc* scene, c
     rc = pc->ensure_stack((size_t)kCutsStack * K * a.n_threads);
     if (rc != PALZ_OK) return rc;
     a.stack = pc->stack;
-    auto walk = [&](bool write) {
-        if (K == 1) launch_walk<1>(a, pc->list[cur], n, pc->scalars + 2, st, write);
-        if (K == 2) launch_walk<2>(a, pc->list[cur], n, pc->scalars + 2, st, write);
-        if (K == 3) launch_walk<3>(a, pc->list[cur], n, pc->scalars + 2, st, write);
+    auto walk = [&](int mode) {
+        if (K == 1) launch_walk<1>(a, pc->list[cur], n, pc->scalars + 2, st, mode);
+        if (K == 2) launch_walk<2>(a, pc->list[cur], n, pc->scalars + 2, st, mode);
+        if (K == 3) launch_walk<3>(a, pc->list[cur], n, pc->scalars + 2, st, mode);
     };
+    if (capacity && capacity <= kCutsOnePassCap) {
+        // one walk into scratch, a stable sort of its words by listed tuple and a
+        // gather; a total past `capacity` falls back to the two-pass walk below
+        rc = pc->ensure_emit(capacity);
+        if (rc != PALZ_OK) return rc;
+        a.out = pc->emit_records;
+        a.words = pc->emit_words;
+        k_clear_total<<<1, 1, 0, st>>>(pc->total);
+        walk(kEmit);
+        PALZ_LAUNCH_CHECK();
+        unsigned long long total = 0;
+        PALZ_HIP(hipMemcpyAsync(&total, pc->total, sizeof(total), hipMemcpyDeviceToHost, st));
+        PALZ_HIP(hipStreamSynchronize(st));
+        a.out = d_paths;
+        a.words = nullptr;
+        if (total <= capacity) {
+            const uint32_t m = (uint32_t)total;
+            int bits = 1;
+            while (bits < 32 && (n - 1u) >> bits) ++bits;
+            if (m > 1) {
+                rc = palz_sort_u64(pc->sorter, pc->emit_words, pc->emit_tmp, nullptr, nullptr, m, 32, 32 + bits, st);
+                if (rc != PALZ_OK) return rc;
+            }
+            if (m) {
+                k_gather_records<<<std::min(div_up(m, 256u), 65536u), 256, 0, st>>>(pc->emit_words, pc->emit_records,
+                                                                                 m, d_paths);
+                PALZ_LAUNCH_CHECK();
+            }
+            k_write_count<<<1, 1, 0, st>>>(d_count, pc->total);
+            PALZ_LAUNCH_CHECK();
+            return PALZ_OK;
+        }
+    }
     k_clear_total<<<1, 1, 0, st>>>(pc->total);
-    walk(false);
+    walk(kCount);
     PALZ_LAUNCH_CHECK();
     launch_exclusive_scan(pc->counts, pc->counts, n, nullptr, d_count, pc->scan_sums, st);
     k_check_total<<<1, 1, 0, st>>>(pc->total, d_count, d_status);
     PALZ_LAUNCH_CHECK();
     if (capacity) {
-        walk(true);
+        walk(kWrite);
         PALZ_LAUNCH_CHECK();
     }
     return PALZ_OK;
