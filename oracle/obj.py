@@ -36,10 +36,10 @@ def load_obj(text: str, default_material: int = 0):
         if not parts:
             continue
         key = parts[0]
-        if key == "v":
-            v.append([f32(float(x)) for x in parts[1:4]])
-        elif key == "vn":
-            vn.append([f32(float(x)) for x in parts[1:4]])
+        if key in ("v", "vn"):
+            if len(parts) < 4:
+                raise ValueError(f"{key} needs three numbers")
+            (v if key == "v" else vn).append([f32(float(x)) for x in parts[1:4]])
         elif key == "usemtl":
             name = parts[1] if len(parts) > 1 else ""
             material = {"mirror": 1, "dielectric": 2}.get(name, default_material)

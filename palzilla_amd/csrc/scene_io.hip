@@ -91,7 +91,8 @@ int load(const char* path, int default_material, palz_mesh& m) {
         while (*s == ' ' || *s == '\t') ++s;
         auto word = [&](const char* w) {
             const size_t n = std::strlen(w);
-            return std::strncmp(s, w, n) == 0 && (s[n] == ' ' || s[n] == '\t');
+            return std::strncmp(s, w, n) == 0 &&
+                   (s[n] == ' ' || s[n] == '\t' || s[n] == '\r' || s[n] == '\n' || s[n] == '\0');
         };
         if (word("v")) {
             s += 1;

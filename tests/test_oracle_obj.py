@@ -48,7 +48,7 @@ def test_corner_normals_and_materials(tmp_path):
     assert a2["material"][1] == T.MATERIAL_DIELECTRIC  # an unknown name takes the default
 
 
-@pytest.mark.parametrize("bad", ["f 1 2 9", "f 1 2", "f 0 1 2", "f 1//3 2 3", "v 1 x 2"])
+@pytest.mark.parametrize("bad", ["f 1 2 9", "f 1 2", "f 0 1 2", "f 1//3 2 3", "v 1 x 2", "v", "vn 1 2"])
 def test_malformed_input_is_an_error(tmp_path, bad):
     with pytest.raises(Exception):
         _native(QUAD + "vn 0 1 0\n" + bad + "\n", tmp_path)
