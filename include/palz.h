@@ -436,6 +436,54 @@ int palz_path_cuts_enumerate(palz_path_cuts* pc, const palz_scene_desc* scene, c
                              uint32_t event, float eta_lo, float eta_hi, uint32_t scene_index, palz_path* d_paths,
                              uint32_t capacity, uint32_t* d_count, uint32_t* d_status, void* stream);
 
+/* ---- renderer: a PKRenderer-shaped host driver (pk.h:1363-1640) ----------
+ * Owns the scene, BVH, photon records, tuple table, path list, batch and the
+ * per-pixel buffers of a W x H image, and calls the entry points above in the
+ * reference's order.  palz_renderer_load_scene / _load_obj = loadFrame (host
+ * arrays in; BVH built); palz_renderer_step = step: solvePrimary, photonTrace
+ * over the eta range of the wavelengths, the tuple table, one lookup of every
+ * chain of length 1-3 and the batched solve (the reference runs one lookup
+ * and launch per descriptor), then the T,T,T,T chains with the plain solver
+ * (the reference's K = 4 launch, pk.h:1563), and the frame's per-wavelength
+ * images resolved (palz_spectral_resolve: CIE fit, `weights`, the frame's
+ * receiver reflectance) into the float4 accumulators; palz_renderer_clear =
+ * clear; palz_renderer_resolve = resolve (pack) into host RGBA8.  The
+ * reference draws one wavelength per pixel and frame; here every frame
+ * solves the fixed list `lambdas` (nm) with quadrature `weights`.  Paths past
+ * path_capacity in a frame are dropped (counts: palz_renderer_buffers). */
+typedef struct {
+    palz_float3 p_light;            /* m_p_light */
+    float light_intensity;          /* 16 (pk.h:1686) */
+    float radiance_clamp;           /* 12 (pk.h:1688) */
+    float min_throughput;           /* 0.05 (pk.h:1687) */
+    float cauchy_a, cauchy_b;       /* CauchyDispersion, lambda in nm */
+    uint32_t photons_per_triangle;  /* 256 (pk.h:1456-1457) */
+    uint32_t use_lens;
+    float lens_distance, lens_thickness, lens_R; /* LensParams (pk.h:1690-1694) */
+    palz_float3 albedo;             /* receiver reflectance */
+    uint32_t path_capacity;         /* paths per frame */
+    int32_t refine;                 /* PALZ_REFINE_* of the batched solve */
+    uint32_t solve_tttt;            /* 1: solve T,T,T,T chains too */
+} palz_renderer_params;
+
+typedef struct palz_renderer palz_renderer;
+int palz_renderer_create(uint32_t width, uint32_t height, const float* lambdas, const float* weights,
+                         uint32_t n_lambdas, palz_renderer** out);
+int palz_renderer_destroy(palz_renderer* r);
+int palz_renderer_set_params(palz_renderer* r, const palz_renderer_params* p);
+int palz_renderer_get_params(const palz_renderer* r, palz_renderer_params* p);
+int palz_renderer_set_camera(palz_renderer* r, const palz_camera* c);
+int palz_renderer_load_scene(palz_renderer* r, const palz_triangle* tris, const palz_triangle_attrib* attribs,
+                             uint32_t n, void* stream);
+int palz_renderer_load_obj(palz_renderer* r, const char* path, int default_material, void* stream);
+int palz_renderer_step(palz_renderer* r, void* stream);
+int palz_renderer_clear(palz_renderer* r, void* stream);
+int palz_renderer_resolve(palz_renderer* r, uint32_t* rgba8, void* stream);
+/* Device views: the last frame's first-diffuse records, the accumulators and
+ * three counts (photon trace status, paths of length 1-3, T,T,T,T paths). */
+int palz_renderer_buffers(const palz_renderer* r, const palz_first_diffuse** d_points, const palz_float4** d_accum,
+                          const uint32_t** d_counts);
+
 /* ---- adjoint of receiver irradiance (no reference counterpart) -----------
  * Heights of a heightfield as parameters: a triangle's vertex c has height
  * parameter tri_vertex[3 t + c] (-1: the triangle is not part of the

@@ -69,6 +69,17 @@ _SIGS = {
                                       C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "palz_solve_specular": (C.c_int, [C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                       C.c_void_p, C.c_void_p]),
+    "palz_renderer_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "palz_renderer_destroy": (C.c_int, [C.c_void_p]),
+    "palz_renderer_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "palz_renderer_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "palz_renderer_set_camera": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "palz_renderer_load_scene": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "palz_renderer_load_obj": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_void_p]),
+    "palz_renderer_step": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "palz_renderer_clear": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "palz_renderer_resolve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "palz_renderer_buffers": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "palz_mesh_load_obj": (C.c_int, [C.c_char_p, C.c_int, C.c_void_p]),
     "palz_mesh_triangle_count": (C.c_int, [C.c_void_p, C.c_void_p]),
     "palz_mesh_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

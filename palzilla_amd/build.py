@@ -19,7 +19,7 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libpalz.so")
 SOURCES = ["runtime.hip", "sort.hip", "lbvh.hip", "prims.hip", "trace.hip", "solve.hip", "cull.hip", "batched.hip", "adjoint.hip",
-           "spectral.hip", "primary.hip", "pathcuts.hip", "scene_io.hip"]
+           "spectral.hip", "primary.hip", "pathcuts.hip", "scene_io.hip", "renderer.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

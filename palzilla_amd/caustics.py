@@ -9,6 +9,7 @@ descriptor -> solveSpecularPath_K.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -430,6 +431,81 @@ class PathCuts:
 
     def status(self) -> int:
         return int(self.d_status.download(np.uint32, 1, self.stream)[0])
+
+
+class RendererParams(C.Structure):
+    """palz_renderer_params."""
+    _fields_ = [("p_light", Float3), ("light_intensity", C.c_float), ("radiance_clamp", C.c_float),
+                ("min_throughput", C.c_float), ("cauchy_a", C.c_float), ("cauchy_b", C.c_float),
+                ("photons_per_triangle", C.c_uint32), ("use_lens", C.c_uint32), ("lens_distance", C.c_float),
+                ("lens_thickness", C.c_float), ("lens_R", C.c_float), ("albedo", Float3),
+                ("path_capacity", C.c_uint32), ("refine", C.c_int32), ("solve_tttt", C.c_uint32)]
+
+
+assert C.sizeof(RendererParams) == 76
+
+
+class Renderer:
+    """palz_renderer_*: the library's PKRenderer-shaped host driver (C++)."""
+
+    def __init__(self, width: int, height: int, lambdas, weights, stream=None):
+        lam = np.ascontiguousarray(lambdas, dtype=np.float32)
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        h = C.c_void_p()
+        call("palz_renderer_create", int(width), int(height), lam.ctypes.data_as(C.c_void_p),
+             w.ctypes.data_as(C.c_void_p), lam.shape[0], C.byref(h))
+        self.handle = h.value
+        self.width, self.height = width, height
+        self.stream = stream
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            try:
+                lib().palz_renderer_destroy(C.c_void_p(self.handle))
+            except Exception:
+                pass
+            self.handle = None
+
+    def params(self) -> RendererParams:
+        p = RendererParams()
+        call("palz_renderer_get_params", C.c_void_p(self.handle), C.byref(p))
+        return p
+
+    def set_params(self, p: RendererParams) -> None:
+        call("palz_renderer_set_params", C.c_void_p(self.handle), C.byref(p))
+
+    def set_camera(self, cam: Camera) -> None:
+        call("palz_renderer_set_camera", C.c_void_p(self.handle), C.byref(cam))
+
+    def load_scene(self, tris: np.ndarray, attribs: np.ndarray) -> None:
+        t = np.ascontiguousarray(tris, dtype=np.float32)
+        a = np.ascontiguousarray(attribs)
+        call("palz_renderer_load_scene", C.c_void_p(self.handle), t.ctypes.data_as(C.c_void_p),
+             a.ctypes.data_as(C.c_void_p), C.c_uint32(t.shape[0]), _sp(self.stream))
+
+    def load_obj(self, path: str, default_material: int = 0) -> None:
+        call("palz_renderer_load_obj", C.c_void_p(self.handle), os.fsencode(path), int(default_material),
+             _sp(self.stream))
+
+    def step(self) -> None:
+        call("palz_renderer_step", C.c_void_p(self.handle), _sp(self.stream))
+
+    def clear(self) -> None:
+        call("palz_renderer_clear", C.c_void_p(self.handle), _sp(self.stream))
+
+    def resolve(self) -> np.ndarray:
+        out = np.zeros(self.width * self.height, dtype=np.uint32)
+        call("palz_renderer_resolve", C.c_void_p(self.handle), out.ctypes.data_as(C.c_void_p), _sp(self.stream))
+        return out
+
+    def buffers(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(first-diffuse records, accumulators (n, 4), counts [status, paths, T,T,T,T paths])."""
+        pts, acc, cnt = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        call("palz_renderer_buffers", C.c_void_p(self.handle), C.byref(pts), C.byref(acc), C.byref(cnt))
+        n = self.width * self.height
+        return (download_ptr(pts.value, FIRST_DIFFUSE, n, self.stream),
+                download_ptr(acc.value, np.float32, 4 * n, self.stream).reshape(n, 4),
+                download_ptr(cnt.value, np.uint32, 3, self.stream))
 
 
 def eta_range(scene: CausticScene) -> tuple[float, float]:

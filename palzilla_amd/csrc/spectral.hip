@@ -20,19 +20,6 @@ namespace {
 
 constexpr int kMaxLambdas = 256;
 
-__device__ __forceinline__ float srgb_oetf(float r) {
-    r = fmaxf(r, 0.0f);
-    if (r <= 0.003130f) return 12.92f * r;
-    return 1.055f * powf(r, 1.0f / 2.4f) - 0.055f;
-}
-__device__ __forceinline__ uint32_t pack_rgba(float r, float g, float b) {
-    auto q = [](float c) -> uint32_t {
-        const int i = (int)(c * 255.0f + 0.5f);  // make_int4: truncation toward zero
-        return (uint32_t)min(max(i, 0), 255);
-    };
-    return 0xFF000000u | (q(b) << 16) | (q(g) << 8) | q(r);
-}
-
 // One thread per receiver point; the per-wavelength linear-sRGB weights are
 // computed once per workgroup into LDS.
 __global__ void __launch_bounds__(256)

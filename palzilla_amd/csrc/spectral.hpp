@@ -106,4 +106,19 @@ PALZ_HD v3 xyz_to_srgb_linear(v3 c) {
               0.0557f * c.x + -0.204f * c.y + 1.057f * c.z);
 }
 
+// The sRGB transfer function and the RGBA8 pack of the reference's `pack`
+// kernel (main_gpu.cu:9-14, :499-513).
+PALZ_HD float srgb_oetf(float r) {
+    r = fmaxf(r, 0.0f);
+    if (r <= 0.003130f) return 12.92f * r;
+    return 1.055f * powf(r, 1.0f / 2.4f) - 0.055f;
+}
+PALZ_HD uint32_t pack_rgba(float r, float g, float b) {
+    auto q = [](float c) -> uint32_t {
+        const int i = (int)(c * 255.0f + 0.5f);  // make_int4: truncation toward zero
+        return (uint32_t)(i < 0 ? 0 : i > 255 ? 255 : i);
+    };
+    return 0xFF000000u | (q(b) << 16) | (q(g) << 8) | q(r);
+}
+
 }  // namespace palz

@@ -1,0 +1,111 @@
+"""The library's PKRenderer-shaped host driver (palz_renderer_*, C++) against
+the oracle composed stage by stage on the CPU: solvePrimary, photon tracing,
+the tuple lookups, the FP32 solve of every chain of length 1-3 plus the
+T,T,T,T chains, the spectral resolve of each frame and the accumulation over
+frames.  The camera pass (first-diffuse records, direct light) is bit-exact;
+the caustic images sum float atomics in any order, so the accumulators are
+compared within rtol 1e-4 and the packed pixels within one step."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import spectral as ospec
+from palzilla_amd import caustics as CA
+from palzilla_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+W = H = 24
+LAMBDAS = np.linspace(400.0, 700.0, 4).astype(np.float32)
+WEIGHTS = np.full(4, 100.0, dtype=np.float32)
+
+
+def _setup():
+    s = scenes.config2(cells=8, width=8, photons=1 << 13, n_lambdas=4)
+    cam = CA.Camera.lookat((0.2, 3.0, 2.2), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 0.7, W, H)
+    r = CA.Renderer(W, H, LAMBDAS, WEIGHTS)
+    p = r.params()
+    p.p_light = CA.Float3(0.0, 3.0, 0.0)
+    p.cauchy_a, p.cauchy_b = 1.5, 4200.0
+    p.photons_per_triangle = 64
+    p.use_lens, p.lens_distance, p.lens_thickness, p.lens_R = 1, 0.3, 0.05, 0.6
+    p.albedo = CA.Float3(0.8, 0.7, 0.6)
+    p.path_capacity = 1 << 20
+    r.set_params(p)
+    r.set_camera(cam)
+    r.load_scene(s.tris, s.attribs)
+    return s, cam, r, p
+
+
+def _oracle_frame(s, cam, p, nodes, root, it):
+    pp = CA.PrimaryParams()
+    pp.p_light = p.p_light
+    pp.light_intensity, pp.radiance_clamp = p.light_intensity, p.radiance_clamp
+    pp.cauchy_a, pp.cauchy_b = p.cauchy_a, p.cauchy_b
+    pp.iteration, pp.width, pp.height = it, W, H
+    pp.use_lens, pp.lens_distance, pp.lens_thickness, pp.lens_R = p.use_lens, p.lens_distance, p.lens_thickness, \
+        p.lens_R
+    pp.albedo = p.albedo
+    fd, direct = oracle.solve_primary(s.tris, s.attribs, nodes, root, cam, pp)
+    f32 = np.float32
+    lo, hi = f32(LAMBDAS.min()), f32(LAMBDAS.max())
+    eta_min = float(f32(p.cauchy_a) + f32(p.cauchy_b) / (lo * lo))
+    eta_max = float(f32(p.cauchy_a) + f32(p.cauchy_b) / (hi * hi))
+    fs = scenes.CausticScene(tris=s.tris, attribs=s.attribs, p_light=(p.p_light.x, p.p_light.y, p.p_light.z),
+                             points=fd, width=W, height=H, extent=s.extent, cauchy_a=p.cauchy_a, cauchy_b=p.cauchy_b,
+                             lambdas=LAMBDAS, photons_per_triangle=p.photons_per_triangle, eta_min=eta_min,
+                             eta_max=eta_max)
+    recs, _ = oracle.photon_trace(fs, nodes, root, iteration=it)
+    paths = oracle.lookup_paths(recs, fd)
+    hs = oracle.HostScene(fs, nodes, root)
+    hs.desc.light_intensity, hs.desc.radiance_clamp = p.light_intensity, p.radiance_clamp
+    oracle.solve_paths(hs, paths[(paths["event"] >> 24) <= 3], threads=8, want_results=False)
+    p4 = oracle.lookup_paths(recs, fd, event=CA.event_u32("TTTT"))
+    if p4.shape[0]:
+        oracle.solve_paths(hs, p4, threads=8, want_results=False)
+    frame, _ = ospec.resolve(hs.image.reshape(LAMBDAS.shape[0], W * H), LAMBDAS, WEIGHTS, fd["R"], 1.0)
+    return fd, direct, frame, paths.shape[0]
+
+
+def test_renderer_matches_the_oracle_over_two_frames():
+    s, cam, r, p = _setup()
+    nodes, root, *_ = oracle.lbvh(s.tris, bits=32)
+    acc = np.zeros((W * H, 4), dtype=np.float32)
+    n_paths = 0
+    for it in range(2):
+        print(f"renderer frame {it}", flush=True)
+        r.step()
+        fd, direct, frame, n = _oracle_frame(s, cam, p, nodes, root, it)
+        n_paths += n
+        got_fd, got_acc, counts = r.buffers()
+        # a miss writes only R = 0 (main_gpu.cu:385); the rest of its record is stale
+        hit = np.any(fd["R"] != 0, axis=1)
+        assert got_fd["R"].tobytes() == fd["R"].tobytes(), f"frame {it}: reflectances differ"
+        assert got_fd[hit].tobytes() == fd[hit].tobytes(), f"frame {it}: first-diffuse records differ"
+        assert hit.sum() > 0
+        assert counts[0] == 0 and counts[1] == n, f"frame {it}: {counts[1]} paths vs the oracle's {n}"
+        acc = acc + direct
+        acc[:, :3] = acc[:, :3] + frame[:, :3]
+        assert np.allclose(got_acc, acc, rtol=1e-4, atol=1e-6), f"frame {it}: accumulators differ"
+    assert n_paths > 0 and np.abs(acc[:, :3]).sum() > 0
+    pix = r.resolve()
+    want = ospec.pack_rgba(ospec.srgb_oetf(acc[:, :3] / acc[:, 3:4]))
+    diff = np.abs(pix.view(np.uint8).reshape(-1, 4).astype(int) - want.view(np.uint8).reshape(-1, 4).astype(int))
+    assert diff.max() <= 1
+    r.clear()
+    assert np.all(r.buffers()[1] == 0)
+
+
+def test_renderer_loads_the_reference_mesh():
+    import os
+    r = CA.Renderer(16, 16, LAMBDAS, WEIGHTS)
+    p = r.params()
+    p.p_light = CA.Float3(0.3, 3.0, 0.2)
+    p.photons_per_triangle = 8
+    p.path_capacity = 1 << 18
+    r.set_params(p)
+    r.set_camera(CA.Camera.lookat((0.0, 2.5, 2.5), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 0.8, 16, 16))
+    r.load_obj(os.path.join(os.path.dirname(__file__), "golden", "table.obj"), 2)
+    r.step()
+    pix = r.resolve()
+    assert pix.shape == (256,) and np.all((pix >> 24) == 0xFF)
