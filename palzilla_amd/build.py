@@ -55,7 +55,25 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    _build_host_program(verbose)
     return LIB
+
+
+HOST_PROGRAM = os.path.join(PKG, "palz_render")
+
+
+def _build_host_program(verbose: bool) -> None:
+    """palz_render: the C++ host program over the renderer, linked against
+    libpalz.so next to it (g++, host code only)."""
+    tmp = HOST_PROGRAM + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "render_main.cpp"), "-o", tmp, "-L", PKG, "-lpalz", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"palz_render build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, HOST_PROGRAM)
 
 
 def clean() -> None:

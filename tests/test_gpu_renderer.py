@@ -109,3 +109,23 @@ def test_renderer_loads_the_reference_mesh():
     r.step()
     pix = r.resolve()
     assert pix.shape == (256,) and np.all((pix >> 24) == 0xFF)
+
+
+def test_cpp_host_program_renders_the_reference_mesh(tmp_path):
+    # palz_render: the C++ host program over the renderer, run as a child
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    prog = os.path.join(root, "palzilla_amd", "palz_render")
+    out = tmp_path / "table.ppm"
+    print("palz_render table.obj 64x64, 2 frames", flush=True)
+    r = subprocess.run([prog, os.path.join(root, "tests", "golden", "table.obj"), str(out), "64", "64", "2"],
+                       capture_output=True, text=True, timeout=120)
+    sys.stdout.write(r.stdout)
+    assert r.returncode == 0, r.stderr
+    data = out.read_bytes()
+    header = b"P6\n64 64\n255\n"
+    assert data.startswith(header) and len(data) == len(header) + 64 * 64 * 3
+    img = np.frombuffer(data[len(header):], dtype=np.uint8).reshape(64, 64, 3)
+    assert img.max() > 0 and "frame 1 done" in r.stdout
