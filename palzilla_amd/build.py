@@ -18,8 +18,10 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libpalz.so")
-SOURCES = ["runtime.hip", "sort.hip", "lbvh.hip", "prims.hip", "trace.hip", "solve.hip", "cull.hip", "batched.hip", "adjoint.hip",
-           "spectral.hip", "primary.hip", "pathcuts.hip", "scene_io.hip", "renderer.hip"]
+SOURCES = [  # the longest compiles first: they run in parallel
+    "pathcuts_k3.hip", "pathcuts_k3_records.hip", "pathcuts_k2.hip", "adjoint.hip", "batched.hip", "lbvh.hip",
+    "pathcuts_k1.hip", "primary.hip", "solve.hip", "cull.hip", "trace.hip", "sort.hip", "prims.hip", "spectral.hip",
+    "runtime.hip", "pathcuts.hip", "scene_io.hip", "renderer.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
