@@ -583,6 +583,48 @@ def bench_path_cuts(args) -> dict:
     return out
 
 
+def bench_renderer(args) -> dict:
+    """The library's PKRenderer-shaped driver (palz_renderer_step) on the C2
+    geometry seen by a 512^2 camera (lens on), 16 wavelengths, 32 photons per
+    triangle (4.2M photons): whole frames, stream-synchronised."""
+    import time
+
+    from palzilla_amd import caustics as CA
+    from palzilla_amd import scenes
+    from palzilla_amd.device import Stream
+
+    s = scenes.config2(cells=256, width=8, photons=1 << 22, n_lambdas=16)
+    st = Stream()
+    lam = s.lambdas.astype(np.float32)
+    w = np.full(lam.shape[0], (float(lam.max()) - float(lam.min())) / (lam.shape[0] - 1), dtype=np.float32)
+    W = H = 512
+    r = CA.Renderer(W, H, lam, w, stream=st)
+    p = r.params()
+    p.p_light = CA.Float3(*s.p_light)
+    p.cauchy_a, p.cauchy_b = s.cauchy_a, s.cauchy_b
+    p.photons_per_triangle = s.photons_per_triangle
+    p.use_lens, p.lens_distance, p.lens_thickness, p.lens_R = 1, 0.3, 0.05, 0.6
+    p.albedo = CA.Float3(0.8, 0.8, 0.8)
+    p.path_capacity = 1 << 26
+    r.set_params(p)
+    r.set_camera(CA.Camera.lookat((0.2, 3.0, 2.2), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 0.9, W, H))
+    r.load_scene(s.tris, s.attribs)
+    r.step()
+    st.sync()
+    frames = 3
+    t = time.perf_counter()
+    for _ in range(frames):
+        r.step()
+    st.sync()
+    ms = (time.perf_counter() - t) * 1e3 / frames
+    _, _, counts = r.buffers()
+    pix = r.resolve()
+    return {"workload": "C2 slab (131,076 tris) seen by a 512^2 camera with the thick lens, 16 wavelengths, "
+                        "32 photons per triangle (4.2M), every chain of length 1-3 batched + T,T,T,T plain",
+            "ms_per_frame": round(ms, 2), "frames_per_s": round(1e3 / ms, 2), "paths_last_frame": int(counts[1]),
+            "lit_pixels": int(np.count_nonzero(pix & 0x00FFFFFF))}
+
+
 def bench_c5(args) -> dict:
     from palzilla_amd import caustics as CA
     from palzilla_amd import scenes
@@ -756,6 +798,7 @@ def main() -> int:
             out["c5"] = bench_c5(args)
             out["lbvh"] = bench_lbvh(args)
             out["path_cuts"] = bench_path_cuts(args)
+            out["renderer"] = bench_renderer(args)
     elif args.workload == "pathcuts":
         r = bench_path_cuts(args)
         out = {"metric": METRIC, "value": r["tuples_per_s"], "unit": "admissible tuples/s", "ms_per_step": r["full_ms"],
