@@ -306,7 +306,17 @@ typedef struct {
     /* Optional (NULL: the nodes alone): the BVH4 view of the same tree from
      * palz_lbvh_widen, used by the batched solver's occlusion rays. */
     const palz_wide_node* wide;
+    /* Optional (NULL: float atomics into `image`): the same sums in unsigned
+     * 32.32 fixed point, each contribution truncated to a multiple of 2^-32.
+     * The sums are integer additions, so they do not depend on the order in
+     * which items finish (the reference's float atomics do); `image` is then
+     * not written.  palz_image_fixed_to_float converts. */
+    uint64_t* image_fixed;
 } palz_scene_desc;
+
+/* image[i] = (float)(fixed[i] * 2^-32), rounded to nearest from the exact
+ * double value, for n entries. */
+int palz_image_fixed_to_float(const uint64_t* d_fixed, float* d_image, size_t n, void* stream);
 
 /* ---- scene ingestion --------------------------------------------------
  * A Wavefront OBJ file as triangles and attributes, with loadFrame's rules
@@ -445,7 +455,7 @@ int palz_path_cuts_enumerate(palz_path_cuts* pc, const palz_scene_desc* scene, c
  * chain of length 1-3 and the batched solve (the reference runs one lookup
  * and launch per descriptor), then the T,T,T,T chains with the plain solver
  * (the reference's K = 4 launch, pk.h:1563), and the frame's per-wavelength
- * images resolved (palz_spectral_resolve: CIE fit, `weights`, the frame's
+ * images (fixed-point sums: independent of the solvers' item order) resolved (palz_spectral_resolve: CIE fit, `weights`, the frame's
  * receiver reflectance) into the float4 accumulators; palz_renderer_clear =
  * clear; palz_renderer_resolve = resolve (pack) into host RGBA8.  The
  * reference draws one wavelength per pixel and frame; here every frame

@@ -207,7 +207,8 @@ def lookup_paths(records: np.ndarray, points: np.ndarray, event: int = 0, step: 
 class HostScene:
     """Host-side SceneDesc of a CausticScene for the oracle solver."""
 
-    def __init__(self, scene, nodes: np.ndarray, root: int):
+    def __init__(self, scene, nodes: np.ndarray, root: int, fixed: bool = False):
+        """fixed: sum contributions as 32.32 fixed point into self.image_fixed."""
         from palzilla_amd import caustics as CA
         self.scene = scene
         self.nodes = np.ascontiguousarray(nodes)
@@ -229,6 +230,10 @@ class HostScene:
         d.radiance_clamp = CA.RADIANCE_CLAMP
         d.cauchy_a = scene.cauchy_a
         d.cauchy_b = scene.cauchy_b
+        self.image_fixed = None
+        if fixed:
+            self.image_fixed = np.zeros(self.image.shape[0], dtype=np.uint64)
+            d.image_fixed = self.image_fixed.ctypes.data
         self.desc = d
 
 

@@ -859,7 +859,8 @@ uint32_t orc_lookup_paths(const palz_photon_record* rec, uint32_t n_rec, float s
 
 // Plain FP32 solve of every (path, wavelength) with `threads` host threads
 // (ParallelFor, main.cpp:1745); adds contributions into sd->image (host) in
-// item order per thread range, and writes results if not NULL.
+// item order, or as 32.32 fixed point into sd->image_fixed when that is set,
+// and writes results if not NULL.
 void orc_solve_paths(const palz_scene_desc* sd, const palz_path* paths, uint32_t n_paths, palz_path_result* results,
                      int threads) {
     SolveCtx c = make_ctx(sd);
@@ -878,9 +879,14 @@ void orc_solve_paths(const palz_scene_desc* sd, const palz_path* paths, uint32_t
             pool.emplace_back(work, items * t / threads, items * (t + 1) / threads);
         for (auto& th : pool) th.join();
     }
-    for (uint64_t it = 0; it < items; it++)
-        if (res[it].status == PALZ_PATH_CONTRIBUTES)
-            sd->image[(size_t)(it % c.n_lambdas) * sd->n_points + paths[it / c.n_lambdas].pixel] += res[it].contribution;
+    for (uint64_t it = 0; it < items; it++) {
+        if (res[it].status != PALZ_PATH_CONTRIBUTES) continue;
+        const size_t idx = (size_t)(it % c.n_lambdas) * sd->n_points + paths[it / c.n_lambdas].pixel;
+        if (sd->image_fixed)  // unsigned 32.32 fixed point, contribution truncated
+            sd->image_fixed[idx] += (uint64_t)(res[it].contribution * 4294967296.0f);
+        else
+            sd->image[idx] += res[it].contribution;
+    }
 }
 
 // FP64 Newton of one chain from `params` (FP32 start), to cost < tol64, with

@@ -57,6 +57,7 @@ _SIGS = {
     "palz_lbvh_widen": (C.c_int, [C.c_void_p, c_void_pp, C.c_void_p]),
     "palz_lbvh_normal_bounds": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "palz_solve_primary": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "palz_image_fixed_to_float": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "palz_spectral_resolve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                         C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
     "palz_photon_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -69,12 +69,12 @@ class SceneDesc(C.Structure):
                 ("points", C.c_void_p), ("lambdas", C.c_void_p), ("image", C.c_void_p),
                 ("n_points", C.c_uint32), ("n_lambdas", C.c_uint32), ("p_light", Float3),
                 ("light_intensity", C.c_float), ("radiance_clamp", C.c_float), ("cauchy_a", C.c_float),
-                ("cauchy_b", C.c_float), ("wide", C.c_void_p)]
+                ("cauchy_b", C.c_float), ("wide", C.c_void_p), ("image_fixed", C.c_void_p)]
 
 
 assert C.sizeof(SceneView) == 40
 assert C.sizeof(PhotonParams) == 32
-assert C.sizeof(SceneDesc) == 104
+assert C.sizeof(SceneDesc) == 112
 
 PHOTON_RECORD = np.dtype([("photon", "<u4"), ("event", "<u4"), ("tris", "<i4", 4), ("params", "<f4", 8),
                           ("p_final", "<f4", 3), ("throughput", "<f4")])
@@ -181,13 +181,14 @@ class GpuScene:
         return SceneView(self.d_tris.ptr, self.d_attribs.ptr, self.bvh.m_internals, self.bvh.m_rootNode,
                          self.scene.n_tris, 0)
 
-    def desc(self, image_ptr: int | None = None, wide: bool = True) -> SceneDesc:
-        """The scene's palz_scene_desc; wide=False leaves out the BVH4 view."""
+    def desc(self, image_ptr: int | None = None, wide: bool = True, image_fixed: int | None = None) -> SceneDesc:
+        """The scene's palz_scene_desc; wide=False leaves out the BVH4 view;
+        image_fixed names a 32.32 fixed-point image to sum into instead."""
         s = self.scene
         return SceneDesc(self.d_tris.ptr, self.d_attribs.ptr, self.bvh.m_internals, self.bvh.m_rootNode,
                          self.d_points.ptr, self.d_lambdas.ptr, image_ptr or self.d_image.ptr, self.n_points,
                          self.n_lambdas, Float3(*s.p_light), LIGHT_INTENSITY, RADIANCE_CLAMP, s.cauchy_a, s.cauchy_b,
-                         self.wide if wide else None)
+                         self.wide if wide else None, image_fixed)
 
     def photon_params(self, iteration: int = 0) -> PhotonParams:
         s = self.scene
@@ -229,8 +230,8 @@ class GpuScene:
         return d_paths, n
 
     def solve_plain(self, K: int, event: int, d_paths: DeviceBuffer, n_paths: int,
-                    d_results: DeviceBuffer | None = None) -> None:
-        d = self.desc()
+                    d_results: DeviceBuffer | None = None, image_fixed: int | None = None) -> None:
+        d = self.desc(image_fixed=image_fixed)
         call("palz_solve_specular", int(K), C.c_uint32(event), C.byref(d), C.c_void_p(d_paths.ptr),
              C.c_uint32(n_paths), C.c_void_p(None), C.c_void_p(d_results.ptr if d_results else None),
              _sp(self.stream))
@@ -241,6 +242,13 @@ class GpuScene:
 
     def clear_image(self) -> None:
         self.d_image.zero(self.stream)
+
+
+def image_fixed_to_float(d_fixed: DeviceBuffer, n: int, stream=None) -> np.ndarray:
+    """palz_image_fixed_to_float of n entries, downloaded."""
+    out = DeviceBuffer(max(n, 1) * 4)
+    call("palz_image_fixed_to_float", C.c_void_p(d_fixed.ptr), C.c_void_p(out.ptr), C.c_size_t(n), _sp(stream))
+    return out.download(np.float32, n, stream)
 
 
 def download_paths(d_paths: DeviceBuffer, n: int) -> np.ndarray:

@@ -444,7 +444,7 @@ __global__ void __launch_bounds__(256)
             const v3 last = chain_vertex(tri[K - 1], pf[P - 2], pf[P - 1]);
             float c = thr * sc->light_intensity / area * hmax(dot(normalize(last - pe), mk(q.ng)), 0.0f);
             c = hmin(c, sc->radiance_clamp);
-            atomicAdd(sc->image + (size_t)l * sc->n_points + path.pixel, c);
+            add_contribution(*sc, (size_t)l * sc->n_points + path.pixel, c);
             r.status = PALZ_PATH_CONTRIBUTES;
             r.contribution = c;
             r.footprint = area;

@@ -236,6 +236,17 @@ PALZ_HD void solve_qr(F* A, F* b, F* x) {
 }
 
 // ---------------------------------------------------------------------------
+// One contribution into a scene's irradiance image: float atomics (the
+// reference's, main_gpu.cu:473-483) or, when the scene names a fixed-point
+// image, a 64-bit integer atomic of c * 2^32 truncated (order-independent).
+__device__ __forceinline__ void add_contribution(const palz_scene_desc& sc, size_t idx, float c) {
+    if (sc.image_fixed)
+        atomicAdd(reinterpret_cast<unsigned long long*>(sc.image_fixed) + idx,
+                  (unsigned long long)(c * 4294967296.0f));
+    else
+        atomicAdd(sc.image + idx, c);
+}
+
 // EventDescriptor (pk.h:122-162): bits [0,24) events (1 = T), [24,32) size.
 PALZ_HD int ev_size(uint32_t e) { return (int)(e >> 24); }
 PALZ_HD bool ev_is_T(uint32_t e, int k) { return (e >> k) & 1u; }

@@ -9,7 +9,9 @@
 //
 // Spectral handling: the reference draws one wavelength per pixel and frame
 // and weights each contribution by the CIE fit at that wavelength.  Here
-// each frame solves a fixed list of wavelengths into per-wavelength images,
+// each frame solves a fixed list of wavelengths into per-wavelength images
+// (32.32 fixed-point sums, so a frame does not depend on the order in which
+// the solvers' items finish),
 // which the spectral resolve weights by the CIE fit, the caller's quadrature
 // weights and the frame's own receiver reflectance before they are added to
 // the accumulators and cleared.
@@ -95,6 +97,7 @@ struct palz_renderer {
     palz_float4* d_frame = nullptr;
     uint32_t* d_pixels = nullptr;
     float* d_images = nullptr;
+    uint64_t* d_fixed = nullptr;  // the solvers' 32.32 fixed-point sums (order-independent)
     float* d_lambdas = nullptr;
     float* d_weights = nullptr;
     palz_scene_desc* d_desc = nullptr;
@@ -124,6 +127,7 @@ struct palz_renderer {
         d.cauchy_a = params.cauchy_a;
         d.cauchy_b = params.cauchy_b;
         d.wide = d_wide;
+        d.image_fixed = d_fixed;
         return d;
     }
     palz_scene_view view() const {
@@ -138,7 +142,8 @@ struct palz_renderer {
     }
     ~palz_renderer() {
         for (void* p : {(void*)d_tris, (void*)d_attribs, (void*)d_points, (void*)d_accum, (void*)d_frame,
-                        (void*)d_pixels, (void*)d_images, (void*)d_lambdas, (void*)d_weights, (void*)d_desc,
+                        (void*)d_pixels, (void*)d_images, (void*)d_fixed, (void*)d_lambdas, (void*)d_weights,
+                        (void*)d_desc,
                         (void*)d_records, (void*)d_paths, (void*)d_scalars})
             if (p) (void)hipFree(p);
         if (batched) palz_batched_destroy(batched);
@@ -194,6 +199,7 @@ int palz_renderer_create(uint32_t width, uint32_t height, const float* lambdas, 
         alloc((void**)&r->d_accum, sizeof(palz_float4) * np) && alloc((void**)&r->d_frame, sizeof(palz_float4) * np) &&
         alloc((void**)&r->d_pixels, sizeof(uint32_t) * np) &&
         alloc((void**)&r->d_images, sizeof(float) * np * n_lambdas) &&
+        alloc((void**)&r->d_fixed, sizeof(uint64_t) * np * n_lambdas) &&
         alloc((void**)&r->d_lambdas, sizeof(float) * n_lambdas) &&
         alloc((void**)&r->d_weights, sizeof(float) * n_lambdas) &&
         alloc((void**)&r->d_desc, sizeof(palz_scene_desc)) && alloc((void**)&r->d_scalars, sizeof(uint32_t) * 4);
@@ -205,7 +211,7 @@ int palz_renderer_create(uint32_t width, uint32_t height, const float* lambdas, 
     if (rc == PALZ_OK) {
         hipStream_t st = nullptr;
         fill_zero(r->d_accum, sizeof(palz_float4) * np, st);
-        fill_zero(r->d_images, sizeof(float) * np * n_lambdas, st);
+        fill_zero(r->d_fixed, sizeof(uint64_t) * np * n_lambdas, st);
         fill_zero(r->d_scalars, sizeof(uint32_t) * 4, st);
         if (hipStreamSynchronize(st) != hipSuccess) rc = fail(PALZ_ERR_HIP, "palz_renderer_create: clear failed");
     }
@@ -362,12 +368,14 @@ int palz_renderer_step(palz_renderer* r, void* stream) {
     }
 
     // the frame's spectral images into the accumulators, then cleared
+    rc = palz_image_fixed_to_float(r->d_fixed, r->d_images, (size_t)np * r->n_lambdas, stream);
+    if (rc != PALZ_OK) return rc;
     rc = palz_spectral_resolve(r->d_images, r->d_lambdas, r->d_weights, r->d_points, np, r->n_lambdas, 1.0f,
                                r->d_frame, nullptr, stream);
     if (rc != PALZ_OK) return rc;
     k_add_frame<<<div_up(np, 256u), 256, 0, st>>>(reinterpret_cast<float4*>(r->d_accum),
                                                   reinterpret_cast<const float4*>(r->d_frame), np);
-    fill_zero(r->d_images, sizeof(float) * np * r->n_lambdas, st);
+    fill_zero(r->d_fixed, sizeof(uint64_t) * np * r->n_lambdas, st);
     PALZ_LAUNCH_CHECK();
     ++r->iteration;
     return PALZ_OK;

@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(kSolveThreads)
             if (contributes) {
                 res.status = PALZ_PATH_CONTRIBUTES;
                 res.contribution = c;
-                atomicAdd(sc.image + (size_t)l * sc.n_points + path.pixel, c);
+                add_contribution(sc, (size_t)l * sc.n_points + path.pixel, c);
             }
         }
 #pragma unroll

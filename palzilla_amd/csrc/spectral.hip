@@ -13,6 +13,7 @@
 // wavelengths, a quadrature weight for a fixed set).
 #include "spectral.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace palz {
@@ -57,8 +58,24 @@ __global__ void __launch_bounds__(256)
     if (pixels) pixels[p] = pack_rgba(srgb_oetf(r / n_frames), srgb_oetf(g / n_frames), srgb_oetf(b / n_frames));
 }
 
+__global__ void __launch_bounds__(256) k_fixed_to_float(const uint64_t* __restrict__ fixed,
+                                                        float* __restrict__ image, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        image[i] = (float)((double)fixed[i] * 0x1p-32);
+}
+
 }  // namespace
 }  // namespace palz
+
+extern "C" int palz_image_fixed_to_float(const uint64_t* d_fixed, float* d_image, size_t n, void* stream) {
+    using namespace palz;
+    if (n && (!d_fixed || !d_image)) return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_image_fixed_to_float: NULL buffer");
+    if (n == 0) return PALZ_OK;
+    const uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 16384);
+    k_fixed_to_float<<<blocks, 256, 0, (hipStream_t)stream>>>(d_fixed, d_image, n);
+    PALZ_LAUNCH_CHECK();
+    return PALZ_OK;
+}
 
 extern "C" int palz_spectral_resolve(const float* d_images, const float* d_lambdas, const float* d_weights,
                                      const palz_first_diffuse* d_points, uint32_t n_points, uint32_t n_lambdas,

@@ -134,6 +134,28 @@ def test_batched_matches_oracle_bit_exact(run):
     assert np.allclose(img, want_img, rtol=1e-5, atol=1e-6)
 
 
+def test_fixed_point_image_is_exact_and_order_independent(run):
+    # 32.32 fixed-point sums: integer additions, so the GPU image equals the
+    # oracle's bit for bit, solve after solve (the float image only to rtol)
+    n_img = run.s.points.shape[0] * run.L
+    keep, want, want_img = run.oracle()
+    hs = oracle.HostScene(run.s, run.nodes, run.root, fixed=True)
+    oracle.solve_paths(hs, run.paths[keep], threads=8, want_results=False)
+    d_fixed = DeviceBuffer(n_img * 8)
+    d_scenes = CA.upload_scene_descs([run.g.desc(image_fixed=d_fixed.ptr)])
+    images = []
+    for graph in (False, True):
+        d_fixed.zero()
+        run.bs.solve(d_scenes, 1, run.L, CA.REFINE_NONE, run.d_res, graph=graph)
+        images.append(d_fixed.download(np.uint64, n_img))
+    assert images[0].tobytes() == hs.image_fixed.tobytes()
+    assert images[1].tobytes() == images[0].tobytes()
+    assert images[0].sum() > 0
+    img = CA.image_fixed_to_float(d_fixed, n_img)
+    assert img.tobytes() == (images[0].astype(np.float64) * 2.0 ** -32).astype(np.float32).tobytes()
+    assert np.allclose(img, want_img, rtol=1e-5, atol=1e-6)
+
+
 def test_wide_nodes_mirror_the_tree(run):
     n_int = run.s.n_tris - 1
     w = download_ptr(run.g.wide, T.WIDE_NODE, n_int)

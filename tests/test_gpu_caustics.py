@@ -128,3 +128,23 @@ def test_plain_solver_bit_exact(pipeline, K):
     assert np.allclose(img, ref, rtol=1e-5, atol=1e-6)
     rel_rmse = np.sqrt(np.mean((img - ref) ** 2)) / max(np.sqrt(np.mean(ref ** 2)), 1e-30)
     assert rel_rmse < 1e-5
+
+
+def test_plain_solver_fixed_point_image_exact(pipeline):
+    # the plain kernel's 32.32 fixed-point image equals the oracle's bit for bit
+    name, s, g, nodes, root = pipeline
+    g.trace()
+    g.build_table()
+    recs = oracle.photon_trace(s, nodes, root)[0]
+    hs = oracle.HostScene(s, nodes, root, fixed=True)
+    L = s.lambdas.shape[0]
+    d_fixed = DeviceBuffer(s.points.shape[0] * L * 8)
+    d_fixed.zero()
+    for ev in sorted(set(int(e) for e in recs["event"] if e)):
+        d_paths, n = g.lookup(ev, capacity=1 << 20)
+        if n == 0:
+            continue
+        g.solve_plain(ev >> 24, ev, d_paths, n, image_fixed=d_fixed.ptr)
+        oracle.solve_paths(hs, CA.download_paths(d_paths, n), threads=8, want_results=False)
+    got = d_fixed.download(np.uint64, s.points.shape[0] * L)
+    assert got.tobytes() == hs.image_fixed.tobytes() and got.sum() > 0

@@ -2,9 +2,10 @@
 the oracle composed stage by stage on the CPU: solvePrimary, photon tracing,
 the tuple lookups, the FP32 solve of every chain of length 1-3 plus the
 T,T,T,T chains, the spectral resolve of each frame and the accumulation over
-frames.  The camera pass (first-diffuse records, direct light) is bit-exact;
-the caustic images sum float atomics in any order, so the accumulators are
-compared within rtol 1e-4 and the packed pixels within one step."""
+frames.  The camera pass (first-diffuse records, direct light) is bit-exact,
+and so are the fixed-point caustic images; the spectral resolve is FP32 code
+on both sides in different operation order, so the accumulators are compared
+within rtol 1e-4 and the packed pixels within one step."""
 import numpy as np
 import pytest
 
@@ -57,13 +58,14 @@ def _oracle_frame(s, cam, p, nodes, root, it):
                              eta_max=eta_max)
     recs, _ = oracle.photon_trace(fs, nodes, root, iteration=it)
     paths = oracle.lookup_paths(recs, fd)
-    hs = oracle.HostScene(fs, nodes, root)
+    hs = oracle.HostScene(fs, nodes, root, fixed=True)
     hs.desc.light_intensity, hs.desc.radiance_clamp = p.light_intensity, p.radiance_clamp
     oracle.solve_paths(hs, paths[(paths["event"] >> 24) <= 3], threads=8, want_results=False)
     p4 = oracle.lookup_paths(recs, fd, event=CA.event_u32("TTTT"))
     if p4.shape[0]:
         oracle.solve_paths(hs, p4, threads=8, want_results=False)
-    frame, _ = ospec.resolve(hs.image.reshape(LAMBDAS.shape[0], W * H), LAMBDAS, WEIGHTS, fd["R"], 1.0)
+    images = (hs.image_fixed.astype(np.float64) * 2.0 ** -32).astype(np.float32)
+    frame, _ = ospec.resolve(images.reshape(LAMBDAS.shape[0], W * H), LAMBDAS, WEIGHTS, fd["R"], 1.0)
     return fd, direct, frame, paths.shape[0]
 
 
