@@ -148,3 +148,45 @@ def test_plain_solver_fixed_point_image_exact(pipeline):
         oracle.solve_paths(hs, CA.download_paths(d_paths, n), threads=8, want_results=False)
     got = d_fixed.download(np.uint64, s.points.shape[0] * L)
     assert got.tobytes() == hs.image_fixed.tobytes() and got.sum() > 0
+
+
+def test_c1_full_size_matches_the_oracle():
+    # the reference's CPU-runnable configuration at its real size (64^2-cell
+    # mirror, 8,192 triangles, 256^2 receiver, 2^16 photons, 550 nm): the
+    # path set, every result field and the fixed-point image equal the oracle's
+    s = scenes.config1()
+    print(f"C1: {s.n_tris} tris, {s.points.shape[0]} receiver points", flush=True)
+    g = CA.GpuScene(s)
+    g.trace()
+    g.build_table()
+    nodes, root = g.bvh.download_nodes(), g.bvh.download_root()
+    recs = oracle.photon_trace(s, nodes, root)[0]
+    want_paths = oracle.lookup_paths(recs, s.points)
+    d_paths, n = g.lookup(0, capacity=max(len(want_paths), 1) + 16)
+    assert n == len(want_paths) and n > 1000
+    paths = CA.download_paths(d_paths, n)
+    assert _canon(paths).tobytes() == _canon(want_paths).tobytes()
+    L = s.lambdas.shape[0]
+    d_fixed = DeviceBuffer(s.points.shape[0] * L * 8)
+    d_fixed.zero()
+    bs = CA.BatchedSolver(g.sorter)
+    bs.prepare(d_paths, n)
+    order = bs.order()
+    d_res = DeviceBuffer(n * L * CA.PATH_RESULT.itemsize)
+    bs.solve(CA.upload_scene_descs([g.desc(image_fixed=d_fixed.ptr)]), 1, L, CA.REFINE_NONE, d_res)
+    res_sorted = CA.download_results(d_res, n * L).reshape(n, L)
+    res = np.empty_like(res_sorted)
+    res[order] = res_sorted
+    res = res.reshape(-1)
+    # chains of length 4 are not part of a batch (reported skipped)
+    keep = np.repeat((paths["event"] >> 24) <= 3, L)
+    assert np.all(res["status"][~keep] == CA.PATH_NOT_CONVERGED) and bs.stats()["skipped"] == int((~keep).sum())
+    hs = oracle.HostScene(s, nodes, root, fixed=True)
+    want = oracle.solve_paths(hs, paths[(paths["event"] >> 24) <= 3], threads=8)
+    got = res[keep]
+    for f in ("status", "iterations", "params", "footprint", "contribution"):
+        a, b = got[f].reshape(got.shape[0], -1), want[f].reshape(want.shape[0], -1)
+        same = np.all((a == b) | (np.isnan(a) & np.isnan(b)), axis=1)
+        assert same.all(), f"C1: field {f} differs in {(~same).sum()} items"
+    assert (want["status"] == CA.PATH_CONTRIBUTES).sum() > 1000
+    assert d_fixed.download(np.uint64, s.points.shape[0] * L).tobytes() == hs.image_fixed.tobytes()
