@@ -360,6 +360,20 @@ def bench_c2(args) -> dict:
                                "frac": round(flops[K] / (kern[f"newton_k{K}"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
                                "flops": flops[K]}
             for K in (1, 2, 3) if kern[f"newton_k{K}"] > 0 and flops[K] > 0}
+    trace = sample.get("photon_trace_cpu")
+    if trace:
+        # photonTrace's algorithmic bytes (64 per node visited, 36 per triangle
+        # tested, 40 per attribute record read, 56 per tuple stored), from the
+        # sampled triangles' counts scaled by the sampling stride
+        w = trace["work"]
+        per_photon = (64 * w["nodes"] + 36 * w["triangles"] + 40 * w["attributes"] + 56 * w["tuples"]) / max(
+            w["photons"], 1)
+        total = per_photon * w["photons"] * trace["stride"]
+        gbps = total / (pipeline_ms["photon_trace"] * 1e-3) / 1e9
+        trace["gpu_roofline"] = {"bound": "hbm (BVH of 8.4 MB largely cache-resident)", "achieved": round(gbps, 1),
+                                 "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4),
+                                 "bytes_per_photon": round(per_photon, 1), "photons_traced": w["photons"] * trace["stride"],
+                                 "kernel_ms": round(pipeline_ms["photon_trace"], 3)}
     setup_s = time.perf_counter() - t_setup
     return {
         "metric": METRIC,
@@ -393,6 +407,7 @@ def bench_c2(args) -> dict:
         "vertex_error": sample["vertex"],
         "parity_sample": sample["parity"],
         "pipeline_ms": {k: round(v, 3) for k, v in pipeline_ms.items()},
+        "photon_trace": sample.get("photon_trace_cpu"),
         "setup_s": round(setup_s, 1),
     }
 
@@ -450,7 +465,17 @@ def verify_sample(args, s, g, bs, d_res, n_all, L, st) -> dict:
         e32.append(np.max(np.linalg.norm(verts(want["params"][it]) - x64, axis=1) / sc))
         e64.append(np.max(np.linalg.norm(verts(res["params"][it]) - x64, axis=1) / sc))
     e32, e64 = np.asarray(e32), np.asarray(e64)
+    # photonTrace on the CPU (the oracle's stackfull traversal, 1 thread) over
+    # every 64th triangle's photons, with its node and triangle counts
+    stride = 64
+    t0 = time.perf_counter()
+    work = oracle.photon_trace_work(s, nodes, root, stride=stride)
+    dt_trace = time.perf_counter() - t0
     return {
+        "photon_trace_cpu": {"value": round(work["photons"] / dt_trace, 1), "unit": "photons/s", "cores": 1,
+                             "kind": "port", "sample": f"oracle photonTrace of every {stride}th triangle's photons "
+                                                       f"({work['photons']} photons, {dt_trace:.2f} s)",
+                             "work": work, "stride": stride},
         "cpu_baseline": {"value": round(cpu_contrib / dt, 1), "unit": "paths/s", "cores": CPU_THREADS,
                          "kind": "port",
                          "sample": f"oracle FP32 solve (solveConstraints + throughput + dA/dw) of "

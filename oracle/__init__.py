@@ -29,6 +29,7 @@ _SIGS = {
     "orc_normal_bounds": (None, [_P, C.c_int, _P, _P]),
     "orc_sobol2d": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "orc_photon_trace": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, _P]),
+    "orc_photon_trace_work": (None, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _P]),
     "orc_lookup_paths": (C.c_uint32, [_P, C.c_uint32, C.c_float, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P,
                                       C.c_uint32, _P]),
     "orc_solve_paths": (None, [_P, _P, C.c_uint32, _P, C.c_int]),
@@ -189,6 +190,21 @@ def photon_trace(scene, nodes: np.ndarray, root: int, iteration: int = 0):
                                    _ptr(np.ascontiguousarray(nodes)), root, C.cast(C.pointer(pp), C.c_void_p),
                                    _ptr(recs))
     return recs, int(depth)
+
+
+def photon_trace_work(scene, nodes: np.ndarray, root: int, stride: int = 1, iteration: int = 0) -> dict:
+    """photonTrace's algorithmic work over the photons of every `stride`-th
+    triangle: nodes visited, triangles tested, attribute reads, tuples stored,
+    photons traced (the reference's stackfull traversal)."""
+    from palzilla_amd import caustics as CA
+    pp = CA.PhotonParams(CA.Float3(*scene.p_light), scene.eta_min, scene.eta_max, iteration, CA.MIN_THROUGHPUT,
+                         scene.photons_per_triangle)
+    out = np.zeros(5, dtype=np.uint64)
+    lib().orc_photon_trace_work(_ptr(np.ascontiguousarray(scene.tris, dtype=np.float32)),
+                                _ptr(np.ascontiguousarray(scene.attribs)), scene.n_tris,
+                                _ptr(np.ascontiguousarray(nodes)), int(root), C.cast(C.pointer(pp), C.c_void_p),
+                                int(stride), _ptr(out))
+    return dict(zip(("nodes", "triangles", "attributes", "tuples", "photons"), (int(v) for v in out)))
 
 
 def lookup_paths(records: np.ndarray, points: np.ndarray, event: int = 0, step: float = 0.01, scene_index: int = 0,

@@ -371,10 +371,17 @@ constexpr uint32_t kLeaf = PALZ_NODE_LEAF_BIT;
 constexpr uint32_t kInvalid = PALZ_NODE_INVALID;
 
 // intersect_stackfull; *max_sp receives the deepest stack use
+// Algorithmic work of photonTrace, counted when set (orc_photon_trace_work):
+// [0] internal nodes visited, [1] triangles tested, [2] attribute records
+// read, [3] tuples stored, [4] photons traced.
+uint64_t* g_work = nullptr;
+uint32_t g_tri_stride = 1;
+
 void intersect_stackfull(Hit* hit, const Scene& s, float3 ro, float3 rd, float3 ird, int* max_sp) {
     std::vector<uint32_t> stack;
     uint32_t node = s.root;
     while (node != kInvalid) {
+        if (g_work) ++g_work[(node & kLeaf) ? 1 : 0];
         if (node & kLeaf) {
             const palz_triangle& tri = s.tris[node & ~kLeaf];
             float t, u, v;
@@ -720,6 +727,7 @@ int orc_photon_trace(const palz_triangle* tris, const palz_triangle_attrib* attr
             std::memset(&rec, 0, sizeof(rec));
             rec.photon = iTri * Pn + j;
             for (int k = 0; k < 4; k++) rec.tris[k] = -1;
+            if (iTri % g_tri_stride) continue;
             if (attribs[iTri].material == PALZ_MATERIAL_DIFFUSE) continue;
             const palz_triangle& tri = tris[iTri];
             float3 v0 = load3(tri.vs[0]);
@@ -743,10 +751,12 @@ int orc_photon_trace(const palz_triangle* tris, const palz_triangle_attrib* attr
             float parameters[8];
             float3 p_final = {0, 0, 0};
             float throughtput = 1.0f;
+            if (g_work) ++g_work[4];
             for (int d = 0; d < 4 + 1; d++) {
                 Hit hit;
                 intersect_stackfull(&hit, s, ro, rd, invRd(rd), &max_sp);
                 if (hit.t == 3.402823466e+38F) break;
+                if (g_work) ++g_work[2];
                 const palz_triangle_attrib& attrib = attribs[hit.tri];
                 float3 p_hit = ro + rd * hit.t;
                 float3 a0 = load3(attrib.shading_normals[0]);
@@ -789,6 +799,7 @@ int orc_photon_trace(const palz_triangle* tris, const palz_triangle_attrib* attr
                 break;
             }
             if (admissiblePath && pp->min_throughput < throughtput) {
+                if (g_work) ++g_work[3];
                 rec.event = ((uint32_t)size << 24) | events;
                 for (int k = 0; k < size; k++) {
                     rec.tris[k] = triIndices[k];
@@ -801,6 +812,22 @@ int orc_photon_trace(const palz_triangle* tris, const palz_triangle_attrib* attr
         }
     }
     return max_sp;
+}
+
+// photonTrace's algorithmic work over the photons of every `stride`-th
+// triangle (the reference's stackfull closest-hit traversal, node by node):
+// counts = {internal nodes visited, triangles tested, attribute records read,
+// tuples stored, photons traced}.
+void orc_photon_trace_work(const palz_triangle* tris, const palz_triangle_attrib* attribs, uint32_t n_tris,
+                           const palz_internal_node* nodes, uint32_t root, const palz_photon_params* pp,
+                           uint32_t stride, uint64_t* counts) {
+    std::vector<palz_photon_record> out((size_t)n_tris * pp->photons_per_triangle);
+    for (int k = 0; k < 5; ++k) counts[k] = 0;
+    g_work = counts;
+    g_tri_stride = stride ? stride : 1;
+    orc_photon_trace(tris, attribs, n_tris, nodes, root, pp, out.data());
+    g_work = nullptr;
+    g_tri_stride = 1;
 }
 
 // The tuple set and the path list, by direct set construction (std::map):
