@@ -440,6 +440,13 @@ def verify_sample(args, s, g, bs, d_res, n_all, L, st) -> dict:
     want = oracle.solve_paths(hs, paths, threads=CPU_THREADS)
     dt = time.perf_counter() - t0
     cpu_contrib = int((want["status"] == CA.PATH_CONTRIBUTES).sum())
+    # the same solve on one thread, over every 8th path of the sample
+    sub = paths[::8]
+    hs1 = oracle.HostScene(s, nodes, root)
+    t0 = time.perf_counter()
+    want1 = oracle.solve_paths(hs1, sub, threads=1)
+    dt1 = time.perf_counter() - t0
+    one_thread = int((want1["status"] == CA.PATH_CONTRIBUTES).sum()) / dt1
     # FP32 parity of the sample: decisions and irradiance equal; params are refined on the GPU
     same = {f: bool(np.array_equal(res[f], want[f])) for f in ("status", "iterations", "contribution",
                                                                  "footprint")}
@@ -477,7 +484,8 @@ def verify_sample(args, s, g, bs, d_res, n_all, L, st) -> dict:
                                                        f"({work['photons']} photons, {dt_trace:.2f} s)",
                              "work": work, "stride": stride},
         "cpu_baseline": {"value": round(cpu_contrib / dt, 1), "unit": "paths/s", "cores": CPU_THREADS,
-                         "kind": "port",
+                         "kind": "port", "host_cores": os.cpu_count(), "value_1_thread": round(one_thread, 1),
+                         "sample_1_thread": f"every 8th path of the sample, {sub.shape[0]} paths, {dt1:.2f} s",
                          "sample": f"oracle FP32 solve (solveConstraints + throughput + dA/dw) of "
                                    f"{paths.shape[0]} paths x {L} wavelengths in {n_chunks} evenly spaced "
                                    f"chunks of the path list, {CPU_THREADS} threads, {dt:.1f} s"},
