@@ -293,6 +293,13 @@ def count_flops(scene, path, p_end, eta: float):
 IV_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4}
 
 
+def iv_square(lo: float, hi: float):
+    """The interval square of [lo, hi] (directed rounding)."""
+    out = np.zeros(2, dtype=np.float32)
+    lib().orc_iv_op(5, float(lo), float(hi), _ptr(out))
+    return float(out[0]), float(out[1])
+
+
 def iv_op(op: str, a: float, b: float = 0.0):
     """(rounded down, rounded up) of a op b in float32 (directed rounding)."""
     out = np.zeros(2, dtype=np.float32)

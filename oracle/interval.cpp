@@ -349,8 +349,15 @@ void cut_walk(const CutScene& sc, uint32_t root, int K, const float* pb, const f
 
 extern "C" {
 
-// op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt (of a); out = {rounded down, rounded up}
+// op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt (of a); out = {rounded down, rounded up};
+// 5: the interval square of [a, b]
 void orc_iv_op(int op, float a, float b, float* out) {
+    if (op == 5) {
+        const Iv r = sq(Iv{a, b});
+        out[0] = r.l;
+        out[1] = r.u;
+        return;
+    }
     switch (op) {
         case 0: out[0] = r_add(a, b, Dir::down); out[1] = r_add(a, b, Dir::up); break;
         case 1: out[0] = r_add(a, -b, Dir::down); out[1] = r_add(a, -b, Dir::up); break;

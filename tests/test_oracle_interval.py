@@ -181,3 +181,42 @@ def test_planted_non_conservative_culling_is_caught():
     bad_keep = np.ones(paths.shape[0], dtype=np.uint8)
     bad_keep[np.nonzero(np.any(status == 2, axis=1))[0][:1]] = 0
     assert np.any(status[bad_keep == 0] == CA.PATH_CONTRIBUTES)
+
+
+class _PCG:
+    """PCG32 as the reference's unittest.cpp:24-51 seeds it."""
+
+    def __init__(self, seed, seq):
+        self.state, self.inc = 0, ((seq << 1) | 1) & 0xFFFFFFFFFFFFFFFF
+        self.uniform()
+        self.state = (self.state + seed) & 0xFFFFFFFFFFFFFFFF
+        self.uniform()
+
+    def uniform(self):
+        old = self.state
+        self.state = (old * 6364136223846793005 + self.inc) & 0xFFFFFFFFFFFFFFFF
+        x = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((x >> rot) | (x << ((-rot) & 31))) & 0xFFFFFFFF
+
+    def uniformf(self):
+        return np.float32((self.uniform() >> 8) / float(1 << 24))  # random_float (sobol.h)
+
+
+def test_square_encloses_and_is_tight_as_the_reference_checks():
+    # unittest.cpp:54-84 with its own inputs: [x, x + 1], x = lerp(-16, 16, u),
+    # 1000 evenly spaced samples per interval.  (The reference folds its upper
+    # sample bound with fminf, unittest.cpp:70, which leaves it at -FLT_MAX;
+    # the intended maximum is used here.)
+    rng = _PCG(103, 178)
+    f32 = np.float32
+    for _ in range(1000):
+        u = rng.uniformf()
+        x = f32(f32(-16.0) + (f32(16.0) - f32(-16.0)) * u)
+        lo_x, hi_x = x, f32(x + f32(1.0))
+        lo, hi = oracle.iv_square(lo_x, hi_x)
+        t = np.arange(1000, dtype=np.float32) / f32(999)
+        xs = (lo_x + (hi_x - lo_x) * t).astype(np.float32)
+        sq = xs * xs
+        assert lo <= sq.min() and sq.max() <= hi
+        assert sq.min() < lo + 0.01 and hi - 0.01 < sq.max()
