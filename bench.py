@@ -555,7 +555,22 @@ def bench_c4(args) -> dict:
     t1.record(st)
     st.sync()
     ms_plain = t0.elapsed_ms(t1) / args.steps
+    # parity at this size: the batched solve and the plain launches sum the same
+    # contributions into per-scene fixed-point images, which must be equal bit for bit
+    fixed_b = [DeviceBuffer(max(g.n_points * L, 1) * 8) for g in batch.scenes]
+    fixed_p = [DeviceBuffer(max(g.n_points * L, 1) * 8) for g in batch.scenes]
+    for b in fixed_b + fixed_p:
+        b.zero(st)
+    d_fx = CA.upload_scene_descs([g.desc(image_fixed=b.ptr) for g, b in zip(batch.scenes, fixed_b)], st)
+    bs.solve(d_fx, len(scene_list), L, CA.REFINE_VERTICES, d_res)
+    index = {id(g): i for i, g in enumerate(batch.scenes)}
+    for gs, K, ev, d, m in plain:
+        gs.solve_plain(K, ev, d, m, image_fixed=fixed_p[index[id(gs)]].ptr)
+    st.sync()
+    equal = all(a.download(np.uint8, a.nbytes, st).tobytes() == b.download(np.uint8, b.nbytes, st).tobytes()
+                for a, b in zip(fixed_b, fixed_p))
     return {"workload": "256 stack scenes, 16^2..256^2 cells, K = 1..3, IOR U[1.3,1.8], 2^14..2^20 photons",
+            "images_equal_batched_vs_plain": equal,
             "paths": int(ranges[3]), "items": int(ranges[3]) * L, "contributing_items": contributing,
             "batched_graph_ms": round(ms, 3), "plain_ms": round(ms_plain, 3), "plain_launches": len(plain),
             "batched_paths_per_s": round(contributing / (ms * 1e-3), 1),
