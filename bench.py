@@ -50,6 +50,10 @@ STAGE_BYTES_PER_TRI = {
     "build_levels": 0.0,  # the levels above the leaves: a few per cent of the nodes
 }
 LBVH_BYTES_PER_TRI = sum(STAGE_BYTES_PER_TRI.values())  # 253 B/tri
+# The same build priced by the reference's separate passes (a count pass reads
+# the keys, the deltas are a pass of their own, the build reads a 4-byte index):
+# 36 + 44 + 8 + 4 x 16 + 9 + 41 per triangle plus 64 per internal node.
+LBVH_REF_PASSES_BYTES = (36 + 44 + 8 + 4 * 16 + 9 + 41, 64)
 LBVH_KERNEL = {"scene_aabb": "k_scene_aabb", "morton_hist": "k_morton_hist", "sort_pass0": "k_sort_pass<false>",
                "sort_pass1": "k_sort_pass<false>", "sort_pass2": "k_sort_pass<false>",
                "sort_pass3": "k_sort_pass<false>", "build": "k_build",
@@ -798,6 +802,9 @@ def bench_lbvh(args) -> dict:
         ms_m = e0.elapsed_ms(e1) / args.steps
         by_size[str(m)] = {"ms_per_build": round(ms_m, 4), "Mtri_per_s": round(m / (ms_m * 1e-3) / 1e6, 1)}
     achieved = LBVH_BYTES_PER_TRI * n / (ms * 1e-3) / 1e9
+    copy_gbps = measure_copy_bandwidth()
+    ref_bytes = LBVH_REF_PASSES_BYTES[0] * n + LBVH_REF_PASSES_BYTES[1] * (n - 1)
+    ref_gbps = ref_bytes / (ms * 1e-3) / 1e9
     out = {"workload": "lbvh+onesweep, 16M-triangle soup (30-bit Morton keys)", "n_triangles": n,
            "value": round(n / (ms * 1e-3) / 1e6, 2), "unit": "Mtri/s", "ms_per_step": round(ms, 4),
            "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
@@ -805,7 +812,11 @@ def bench_lbvh(args) -> dict:
            "hbm_frac": round(achieved / HBM_PEAK_GBPS, 4),
            "stage_GBps": {k: round(STAGE_BYTES_PER_TRI[k] * n / (v * 1e-3) / 1e9, 1)
                           for k, v in stage_ms.items() if v > 0 and STAGE_BYTES_PER_TRI[k] > 0},
-           "copy_kernel_GBps": round(measure_copy_bandwidth(), 1), "smaller_sizes": by_size}
+           "copy_kernel_GBps": round(copy_gbps, 1),
+           "reference_pass_bytes": {"bytes": ref_bytes, "GBps": round(ref_gbps, 1),
+                                    "frac_of_8TBps": round(ref_gbps / HBM_PEAK_GBPS, 4),
+                                    "frac_of_copy_kernel": round(ref_gbps / copy_gbps, 4)},
+           "smaller_sizes": by_size}
     if not args.no_cpu:
         import oracle
         sample = min(n, args.cpu_sample)
