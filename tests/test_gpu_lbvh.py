@@ -189,3 +189,25 @@ def test_normal_bounds_bit_exact(sorter, n, option, run):
     want = oracle.normal_bounds(b.download_nodes(), attribs)
     assert got.tobytes() == want.tobytes()
     assert np.all(got["counter"] == 2)
+
+
+@pytest.fixture(scope="module")
+def benchmark_soup():
+    # the LBVH benchmark's 16M-triangle soup; its 4M build is the first 4M
+    return scenes.triangle_soup(1 << 24, seed=2024)
+
+
+@pytest.mark.parametrize("n", [1 << 22, 1 << 24])
+def test_lbvh_bit_exact_benchmark_sizes(sorter, benchmark_soup, n):
+    """The benchmark's own 4M and 16M builds (30-bit keys), bit for bit against
+    the oracle: scene box, sorted words, deltas, root and every node."""
+    print(f"lbvh benchmark size n={n}", flush=True)
+    tris = np.ascontiguousarray(benchmark_soup[:n])
+    nodes, root, words, deltas, box = oracle.lbvh(tris, bits=30)
+    b = _gpu_lbvh(sorter, tris, 30)
+    assert b.download_scene_aabb().tobytes() == box.tobytes()
+    g_words, g_deltas = b.download_sorted()
+    assert np.array_equal(g_words, words)
+    assert np.array_equal(g_deltas, deltas)
+    assert b.download_root() == root
+    assert b.download_nodes().tobytes() == nodes.tobytes()
