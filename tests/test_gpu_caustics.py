@@ -190,3 +190,56 @@ def test_c1_full_size_matches_the_oracle():
         assert same.all(), f"C1: field {f} differs in {(~same).sum()} items"
     assert (want["status"] == CA.PATH_CONTRIBUTES).sum() > 1000
     assert d_fixed.download(np.uint64, s.points.shape[0] * L).tobytes() == hs.image_fixed.tobytes()
+
+
+def test_c2_full_size_photons_exact_and_sampled_points_match_the_oracle():
+    # the spectral slab at its real size (256^2-cell wavy top, 131,076
+    # triangles, 2^22 photons, 16 wavelengths): every photon record bit for
+    # bit; then, on every 97th point of the 1024^2 receiver, the path list,
+    # every result field and the fixed-point image equal the oracle's
+    import dataclasses
+
+    s = scenes.config2()
+    print(f"C2: {s.n_tris} tris, {s.photons_per_triangle} photons/tri", flush=True)
+    g = CA.GpuScene(s)
+    g.trace()
+    assert g.trace_status() == 0
+    nodes, root = g.bvh.download_nodes(), g.bvh.download_root()
+    recs, depth = oracle.photon_trace(s, nodes, root)
+    assert depth <= 64 and (recs["event"] != 0).sum() > 1_000_000
+    assert g.records().tobytes() == recs.tobytes()
+    del g
+    sub = dataclasses.replace(s, points=np.ascontiguousarray(s.points[::97]))
+    print(f"C2 receiver sample: {sub.points.shape[0]} points", flush=True)
+    gs = CA.GpuScene(sub)
+    gs.trace()
+    gs.build_table()
+    assert gs.records().tobytes() == recs.tobytes()
+    want_paths = oracle.lookup_paths(recs, sub.points)
+    d_paths, n = gs.lookup(0, capacity=len(want_paths) + 16)
+    assert n == len(want_paths) and n > 100_000
+    paths = CA.download_paths(d_paths, n)
+    assert _canon(paths).tobytes() == _canon(want_paths).tobytes()
+    L = s.lambdas.shape[0]
+    d_fixed = DeviceBuffer(sub.points.shape[0] * L * 8)
+    d_fixed.zero()
+    bs = CA.BatchedSolver(gs.sorter)
+    bs.prepare(d_paths, n)
+    order = bs.order()
+    d_res = DeviceBuffer(n * L * CA.PATH_RESULT.itemsize)
+    bs.solve(CA.upload_scene_descs([gs.desc(image_fixed=d_fixed.ptr)]), 1, L, CA.REFINE_NONE, d_res)
+    res_sorted = CA.download_results(d_res, n * L).reshape(n, L)
+    res = np.empty_like(res_sorted)
+    res[order] = res_sorted
+    res = res.reshape(-1)
+    keep = np.repeat((paths["event"] >> 24) <= 3, L)
+    assert bs.stats()["skipped"] == int((~keep).sum())
+    hs = oracle.HostScene(sub, nodes, root, fixed=True)
+    want = oracle.solve_paths(hs, paths[(paths["event"] >> 24) <= 3], threads=16)
+    got = res[keep]
+    for f in ("status", "iterations", "params", "footprint", "contribution"):
+        a, b = got[f].reshape(got.shape[0], -1), want[f].reshape(want.shape[0], -1)
+        same = np.all((a == b) | (np.isnan(a) & np.isnan(b)), axis=1)
+        assert same.all(), f"C2: field {f} differs in {(~same).sum()} items"
+    assert (want["status"] == CA.PATH_CONTRIBUTES).sum() > 100_000
+    assert d_fixed.download(np.uint64, sub.points.shape[0] * L).tobytes() == hs.image_fixed.tobytes()
