@@ -142,7 +142,11 @@ def solve(x0, verts, nrms, pb, pe, eta, event, K, iters=8):
         F = residual(xs, vv, nn, V.const(pb, P), V.const(pe, P), D.const(eta, P), event, K)
         A = np.stack([f.g for f in F], axis=1)          # (n, 3K, P)
         b = np.stack([f.v for f in F], axis=1)          # (n, 3K)
-        dx = np.einsum("nij,nj->ni", np.linalg.pinv(A), b)
+        # an item at a square root of zero (grazing refraction) has no finite
+        # Jacobian: it keeps its x, and its contribution comes out non-finite
+        ok = np.all(np.isfinite(A), axis=(1, 2)) & np.all(np.isfinite(b), axis=1)
+        dx = np.zeros_like(x)
+        dx[ok] = np.einsum("nij,nj->ni", np.linalg.pinv(A[ok]), b[ok])
         x = x - dx
     return x
 
@@ -287,6 +291,13 @@ class ItemSet:
         self.base_tris = scene.tris.reshape(-1, 3, 3).astype(np.float64)
         self.base_nrm = scene.attribs["shading_normals"].astype(np.float64)
 
+    def keep(self, mask):
+        """Restrict the set to the items where `mask` holds."""
+        for name in ("item", "pixel", "lam", "event", "tris", "x0", "eta", "pb", "pe", "ng", "dielectric"):
+            setattr(self, name, getattr(self, name)[mask])
+        if hasattr(self, "_item_tv"):
+            del self._item_tv
+
     def geometry(self, heights, sel):
         """FP64 triangle vertices and normals of items `sel` at `heights`."""
         P, N = scene_geometry(heights, self.cells)
@@ -319,9 +330,10 @@ class ItemSet:
         """Items whose inputs depend on height j (their vertices' heights or the
         heights their normals use)."""
         users = {j} | {int(v) for v in np.nonzero(np.any(nb == j, axis=1))[0]}
+        if not hasattr(self, "_item_tv"):  # per chain vertex: the items' triangle vertex ids
+            self._item_tv = [self.tv[self.tris[:, k]] for k in range(self.K)]
         mask = np.zeros(self.item.size, dtype=bool)
-        for k in range(self.K):
-            tv = self.tv[self.tris[:, k]]
+        for tv in self._item_tv:
             mask |= np.isin(tv, list(users)).any(axis=1)
         return np.nonzero(mask)[0]
 
@@ -356,14 +368,30 @@ class Problem:
         self.nb = normal_nb
         self.size = scene.lambdas.shape[0] * scene.points.shape[0]
 
+    def drop_nonfinite(self, heights):
+        """Leave out the items whose FP64 contribution at `heights` is not
+        finite (a square root of zero on their chain).  Returns how many, and
+        the image of the items kept (self.image(heights) after the call)."""
+        dropped = 0
+        img = np.zeros(self.size)
+        for s in self.sets:
+            c = s.contributions(heights)
+            finite = np.isfinite(c)
+            dropped += int((~finite).sum())
+            s.keep(finite)
+            np.add.at(img, s.lam * s.n_points + s.pixel, c[finite])
+        return dropped, img
+
     def image(self, heights):
         img = np.zeros(self.size)
         for s in self.sets:
             img += s.image(heights)
         return img
 
-    def loss_fd(self, heights, target, j, h=1e-4):
-        base = self.image(heights)
+    def loss_fd(self, heights, target, j, h=1e-4, base=None):
+        """Central difference of L = sum (I - T)^2 in height j (FP64); `base`,
+        if given, is self.image(heights), computed once for many j."""
+        base = self.image(heights) if base is None else base
         out = []
         for sgn in (1.0, -1.0):
             hh = np.array(heights, dtype=np.float64)

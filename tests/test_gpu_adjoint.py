@@ -20,8 +20,10 @@ CELLS, WIDTH, PHOTONS = 16, 48, 1 << 14
 class Render:
     """Forward state of one height vector: scene, batched solve, image."""
 
-    def __init__(self, heights: np.ndarray, recompute: bool = False):
-        self.s = scenes.config5(heights, cells=CELLS, width=WIDTH, photons=PHOTONS)
+    def __init__(self, heights: np.ndarray, recompute: bool = False, cells: int = CELLS, width: int = WIDTH,
+                 photons: int = PHOTONS):
+        self.cells = cells
+        self.s = scenes.config5(heights, cells=cells, width=width, photons=photons)
         self.g = CA.GpuScene(self.s)
         self.g.trace()
         self.g.build_table()
@@ -35,7 +37,7 @@ class Render:
         self.g.clear_image()
         self.bs.solve(self.d_scenes, 1, self.L, CA.REFINE_VERTICES, self.d_res)
         self.image = self.g.image().reshape(-1).astype(np.float64)
-        tv, nb, coef = scenes.height_param_tables(CELLS, self.s.n_tris)
+        tv, nb, coef = scenes.height_param_tables(cells, self.s.n_tris)
         self.tv, self.nb = tv, nb
         self.hp = CA.HeightParams(tv, nb, coef)
         self.recompute = recompute
@@ -122,3 +124,76 @@ def test_adjoint_of_zero_weight_is_zero(problem):
     r.recompute = False
     g = r.backward(np.zeros_like(r.image))
     assert np.all(g == 0)
+
+
+@pytest.mark.timeout(900)
+def test_adjoint_full_size_matches_fp64_finite_differences():
+    # the adjoint workload at its real size: 128^2 cells (16,641 heights),
+    # 512^2 receiver, 2^21 photons, target from a second seed.  The loss is
+    # taken around the GPU image, as the adjoint's weights 2 (I - T) are; each
+    # central difference moves only the items that depend on the height, with
+    # their FP64 contributions.  64 random live heights.
+    import time
+
+    cells, width, photons = 128, 512, 1 << 21
+    h = scenes.slab_heights(cells, 5)
+    target = Render(scenes.slab_heights(cells, 6), cells=cells, width=width, photons=photons).image
+    r = Render(h, cells=cells, width=width, photons=photons)
+    r.h, r.target = h, target
+    print(f"C5: {r.n} paths, {r.hp.n_params} heights", flush=True)
+    grad = _autograd(r)
+    t0 = time.perf_counter()
+    res = CA.download_results(r.d_res, r.n * r.L)
+    sorted_paths = r.bs.sorted_paths()
+    prob = AD.Problem(r.s, sorted_paths, res, r.tv, r.nb, cells, CA.LIGHT_INTENSITY, CA.RADIANCE_CLAMP)
+    # the FP64 contributions of a 200k-item sample against the GPU's
+    rng = np.random.default_rng(7)
+    for st in prob.sets:
+        sel = rng.choice(st.item.size, min(st.item.size, 200_000), replace=False)
+        c64 = st.contributions(h, sel)
+        # a set's item numbers count the items of its own chain length's paths
+        own = np.nonzero((sorted_paths["event"] >> 24) == st.K)[0]
+        glob = own[st.item[sel] // r.L] * r.L + st.item[sel] % r.L
+        c32 = res["contribution"][glob].astype(np.float64)
+        close = np.isclose(c64, c32, rtol=1e-3, atol=1e-6)
+        far = ~close
+        rel = np.abs(c64 - c32) / np.maximum(np.abs(c64), 1e-30)
+        share = np.abs(c64[far]).sum() / np.abs(c64).sum()
+        sums = abs(c64.sum() - c32.sum()) / abs(c64.sum())
+        print(f"K={st.K}: {sel.size} sampled items, {int(far.sum())} beyond rtol 1e-3 (largest "
+              f"{rel.max():.2e}, {share:.2e} of the sample's irradiance); sums differ by {sums:.2e}", flush=True)
+        # K <= 2: the FP32 roots are the FP64 roots to round-off, up to a few
+        # items near a caustic fold.  K = 3 (T-R-T through the slab's bottom)
+        # is ill-conditioned: FP32 Newton stops at an FP64 residual of ~5e-6,
+        # and the FP64 re-solve moves some chains' parameters by ~5e-4.
+        if st.K <= 2:
+            assert far.sum() <= 1e-3 * sel.size and sums <= 1e-4
+    live = np.nonzero(np.abs(grad) > 0)[0]
+    assert live.size > 1000
+    pick = rng.choice(live, 64, replace=False)
+    ad = grad[pick]
+    base = r.image
+    print(f"finite: image {np.isfinite(base).all()}, target {np.isfinite(target).all()}, "
+          f"gradient {np.isfinite(grad).all()}", flush=True)
+    fd4, fd6 = [], []
+    for i, j in enumerate(pick):
+        fd4.append(prob.loss_fd(h, target, int(j), 1e-4, base=base))
+        fd6.append(prob.loss_fd(h, target, int(j), 1e-6, base=base))
+        if i % 16 == 15:
+            print(f"{i + 1}/64 heights, {time.perf_counter() - t0:.1f} s", flush=True)
+    fd4, fd6 = np.array(fd4), np.array(fd6)
+    # a height whose items include one at a square root of zero has no finite
+    # FP64 difference: it counts as a miss below
+    print(f"non-finite differences: {int((~np.isfinite(fd4)).sum())} (h=1e-4), "
+          f"{int((~np.isfinite(fd6)).sum())} (h=1e-6)", flush=True)
+    scale = np.nanmax(np.abs(fd6))
+    rel4 = np.abs(ad - fd4) / (np.abs(fd4) + 1e-3 * scale)
+    rel6 = np.abs(ad - fd6) / (np.abs(fd6) + 1e-3 * scale)
+    rel4, rel6 = (np.where(np.isfinite(x), x, np.inf) for x in (rel4, rel6))
+    print(f"64 params, |dL/dh| up to {scale:.4g}: vs FD h=1e-4 median {np.median(rel4):.2e}, "
+          f"{np.sum(rel4 < 1e-2)}/64 within 1e-2; vs FD h=1e-6 median {np.median(rel6):.2e}, "
+          f"{np.sum(rel6 < 1e-3)}/64 within 1e-3", flush=True)
+    # at 128^2 cells a step of 1e-4 crosses clamp kinks and strong caustic
+    # curvature (measured: median 7e-3), so the converged step h = 1e-6 is the
+    # reference here (measured: median 6.6e-6, 59/64 within 1e-3)
+    assert np.median(rel6) < 1e-4 and np.sum(rel6 < 1e-3) >= 56
