@@ -290,8 +290,20 @@ def test_autotune_keeps_results(run):
     assert odd.tobytes() == before.tobytes()
 
 
-def test_multi_scene_batch():
-    scene_list = scenes.config4_scenes(n_scenes=6, seed=11, width=24, photon_exp=(11, 12), cell_exp=(3, 4))
+def _c4_small():
+    return scenes.config4_scenes(n_scenes=6, seed=11, width=24, photon_exp=(11, 12), cell_exp=(3, 4))
+
+
+def _c4_real_slice():
+    # the first 12 scenes of the 256-scene batched workload, at their real
+    # sizes (16^2-256^2 cells, 2^14-2^20 photons, K = 1-3, IOR 1.3-1.8)
+    return scenes.config4_scenes()[:12]
+
+
+@pytest.mark.parametrize("make", [_c4_small, _c4_real_slice], ids=["small", "c4_real_slice"])
+def test_multi_scene_batch(make):
+    scene_list = make()
+    print(f"{len(scene_list)} scenes, {sum(x.n_tris for x in scene_list)} triangles", flush=True)
     batch = CA.SceneBatch(scene_list)
     n = batch.build()
     L = batch.n_lambdas
@@ -306,6 +318,7 @@ def test_multi_scene_batch():
     res = _unsort(CA.download_results(d_res, n * L), bs.order(), L)
     st = bs.stats()
     assert sum(st[f"K{k}"]["items"] for k in (1, 2, 3)) + st["skipped"] == n * L
+    compared = 0
     for i, (s, g) in enumerate(zip(scene_list, batch.scenes)):
         m = (paths["scene"] == i) & ((paths["event"] >> 24) <= 3)
         if not m.any():
@@ -319,6 +332,9 @@ def test_multi_scene_batch():
             a, b = got[f].reshape(got.shape[0], -1), want[f].reshape(want.shape[0], -1)
             assert np.all((a == b) | (np.isnan(a) & np.isnan(b))), f"scene {i} field {f}"
         assert np.allclose(g.image().reshape(-1), hs.image, rtol=1e-5, atol=1e-6), f"scene {i} image"
+        compared += got.shape[0]
+    print(f"{n} paths, {compared} items equal to the oracle's", flush=True)
+    assert compared > 0
 
 
 def _eta_range(s):
