@@ -47,6 +47,15 @@ _SIGS = {
     "orc_cmf_y_sample": (C.c_float, [C.c_float]),
     "orc_cmf_y_pdf": (C.c_float, [C.c_float]),
     "orc_solve_primary": (None, [_P, _P, _P, C.c_uint32, _P, _P, _P, _P, _P]),
+    "orc_fresnel": (C.c_float, [_P, _P, C.c_float]),
+    "orc_refraction": (C.c_int, [_P, _P, C.c_float, _P]),
+    "orc_dual_optics": (None, [C.c_int, _P, _P, C.c_float, _P]),
+    "orc_iv_ops": (None, [C.c_int, _P, _P, _P]),
+    "orc_ray_triangle": (C.c_int, [_P, _P, _P, C.c_float, C.c_float, _P]),
+    "orc_closest_hits": (None, [_P, _P, C.c_uint32, _P, _P, C.c_uint32, _P, _P]),
+    "orc_brute_hits": (None, [_P, C.c_uint32, _P, _P, C.c_uint32, C.c_float, _P, _P, _P]),
+    "orc_any_hits": (None, [_P, _P, C.c_uint32, _P, _P, C.c_uint32, C.c_float, _P]),
+    "orc_occluded": (None, [_P, _P, C.c_uint32, _P, _P, _P, _P, C.c_uint32, _P]),
 }
 
 _lib = None
@@ -351,3 +360,111 @@ def path_cuts(tris, attribs, nodes, root: int, nbounds, points, p_light, event: 
     tuples = np.zeros((max(total, 1), K), dtype=np.int32)
     lib().orc_path_cuts(*args, _ptr(tuples), total)
     return counts[:n], tuples[:total]
+
+
+# ---- single-function entry points used by the optics and traversal pins ----
+
+def fresnel(wi, n, eta: float) -> float:
+    wi = np.asarray(wi, np.float32)
+    n = np.asarray(n, np.float32)
+    return float(lib().orc_fresnel(_ptr(wi), _ptr(n), eta))
+
+
+def refraction(wi, n, eta: float):
+    """Norm-free refraction (pk.h:181-200): the direction, or None on TIR."""
+    wi = np.asarray(wi, np.float32)
+    n = np.asarray(n, np.float32)
+    wo = np.zeros(3, np.float32)
+    return wo if lib().orc_refraction(_ptr(wi), _ptr(n), eta, _ptr(wo)) else None
+
+
+def dual_optics(op: int, wi6, n6, eta: float = 1.0) -> np.ndarray:
+    """One-tangent dual reflection (op 0) / refraction (op 1), {v.xyz, g.xyz}."""
+    wi6 = np.ascontiguousarray(wi6, np.float32)
+    n6 = np.ascontiguousarray(n6, np.float32)
+    out = np.zeros(6, np.float32)
+    lib().orc_dual_optics(op, _ptr(wi6), _ptr(n6), eta, _ptr(out))
+    return out
+
+
+def iv_ops(op: int, a, b=None) -> np.ndarray:
+    """Directed-rounding interval ops in interval.h's operand layout."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(np.zeros(6) if b is None else b, np.float32)
+    out = np.zeros(6, np.float32)
+    lib().orc_iv_ops(op, _ptr(a), _ptr(b), _ptr(out))
+    return out[:6] if op == 7 else out[:2]
+
+
+def ray_triangle(tri: np.ndarray, ro, rd, t_min: float = 0.0, t_max: float = 3.402823466e38):
+    """(t, u, v, ng) of intersectRayTriangle, or None on a miss."""
+    tri = np.ascontiguousarray(tri, np.float32).reshape(9)
+    ro = np.ascontiguousarray(ro, np.float32)
+    rd = np.ascontiguousarray(rd, np.float32)
+    out = np.zeros(6, np.float32)
+    if not lib().orc_ray_triangle(_ptr(tri), _ptr(ro), _ptr(rd), t_min, t_max, _ptr(out)):
+        return None
+    return out
+
+
+def closest_hits(tris, nodes, root: int, ro, rd):
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    ro = np.ascontiguousarray(ro, np.float32).reshape(-1, 3)
+    rd = np.ascontiguousarray(rd, np.float32).reshape(-1, 3)
+    n = ro.shape[0]
+    t = np.zeros(n, np.float32)
+    tri = np.zeros(n, np.uint32)
+    lib().orc_closest_hits(_ptr(tris), _ptr(nodes), root, _ptr(ro), _ptr(rd), n, _ptr(t), _ptr(tri))
+    return t, tri
+
+
+def brute_hits(tris, ro, rd, t_max: float = 1.0):
+    """Every triangle tested, no BVH: (nearest t, its triangle, any hit in [0, t_max])."""
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    ro = np.ascontiguousarray(ro, np.float32).reshape(-1, 3)
+    rd = np.ascontiguousarray(rd, np.float32).reshape(-1, 3)
+    n = ro.shape[0]
+    t = np.zeros(n, np.float32)
+    tri = np.zeros(n, np.uint32)
+    anyh = np.zeros(n, np.uint8)
+    lib().orc_brute_hits(_ptr(tris), tris.shape[0], _ptr(ro), _ptr(rd), n, t_max, _ptr(t), _ptr(tri), _ptr(anyh))
+    return t, tri, anyh.astype(bool)
+
+
+def any_hits(tris, nodes, root: int, ro, rd, t_max: float = 1.0) -> np.ndarray:
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    ro = np.ascontiguousarray(ro, np.float32).reshape(-1, 3)
+    rd = np.ascontiguousarray(rd, np.float32).reshape(-1, 3)
+    out = np.zeros(ro.shape[0], np.uint8)
+    lib().orc_any_hits(_ptr(tris), _ptr(nodes), root, _ptr(ro), _ptr(rd), ro.shape[0], t_max, _ptr(out))
+    return out.astype(bool)
+
+
+def occluded(tris, nodes, root: int, p_from, n_from, p_to, n_to) -> np.ndarray:
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    args = [np.ascontiguousarray(a, np.float32).reshape(-1, 3) for a in (p_from, n_from, p_to, n_to)]
+    out = np.zeros(args[0].shape[0], np.uint8)
+    lib().orc_occluded(_ptr(tris), _ptr(nodes), root, *[_ptr(a) for a in args], args[0].shape[0], _ptr(out))
+    return out.astype(bool)
+
+
+# ---- the reference's own standalone headers, built by oracle/ref_build.py ----
+
+REF_LIB_PATH = os.path.join(HERE, "_ref", "libref.so")
+_ref = None
+
+
+def ref_lib() -> C.CDLL | None:
+    """oracle/_ref/libref.so (sobol.h, saka.h, interval.h compiled from the
+    reference tree), or None where it was not built (no reference tree)."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB_PATH):
+        h = C.CDLL(REF_LIB_PATH)
+        h.ref_sobol2d.restype = None
+        h.ref_sobol2d.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]
+        h.ref_saka_optics.restype = None
+        h.ref_saka_optics.argtypes = [C.c_int, _P, _P, C.c_float, _P]
+        h.ref_interval.restype = None
+        h.ref_interval.argtypes = [C.c_int, _P, _P, _P]
+        _ref = h
+    return _ref

@@ -367,6 +367,32 @@ void orc_iv_op(int op, float a, float b, float* out) {
     }
 }
 
+// Interval operations on {l, u} pairs and 6-float boxes {x.l, x.u, y.l, y.u,
+// z.l, z.u}, in the operand layout of the reference's interval.h: op 0 +,
+// 1 -, 2 *, 3 /, 4 sqrt(a), 5 square(a), 6 dot, 7 cross (out[0..5]),
+// 8 lengthSquared(a).
+void orc_iv_ops(int op, const float* a, const float* b, float* out) {
+    auto box = [](const float* p) { return Iv3{{{p[0], p[1]}, {p[2], p[3]}, {p[4], p[5]}}}; };
+    Iv r;
+    switch (op) {
+        case 0: r = Iv{a[0], a[1]} + Iv{b[0], b[1]}; break;
+        case 1: r = Iv{a[0], a[1]} - Iv{b[0], b[1]}; break;
+        case 2: r = Iv{a[0], a[1]} * Iv{b[0], b[1]}; break;
+        case 3: r = Iv{a[0], a[1]} / Iv{b[0], b[1]}; break;
+        case 4: r = root(Iv{a[0], a[1]}); break;
+        case 5: r = sq(Iv{a[0], a[1]}); break;
+        case 6: r = dot3(box(a), box(b)); break;
+        case 7: {
+            const Iv3 c = cross3(box(a), box(b));
+            for (int i = 0; i < 3; ++i) out[2 * i] = c.c[i].l, out[2 * i + 1] = c.c[i].u;
+            return;
+        }
+        default: r = norm2(box(a)); break;
+    }
+    out[0] = r.l;
+    out[1] = r.u;
+}
+
 // Enclosure of the residual c_k = cross(wo, wopt) of one oriented vertex over
 // boxes (6 floats each: lo xyz, hi xyz): out = 3 x {l, u}; returns 0 when the
 // refraction box may hold total internal reflection (no enclosure computed).

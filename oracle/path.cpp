@@ -1254,4 +1254,99 @@ void orc_solve_primary(const palz_triangle* tris, const palz_triangle_attrib* at
 float orc_cmf_y_sample(float u) { return cmfYSample(u); }
 float orc_cmf_y_pdf(float x) { return cmfYPdf(x); }
 
+// ---- single-function entry points for the pins of the optics and traversal --
+
+// fresnel_exact_norm_free (pk.h:427-450)
+float orc_fresnel(const float* wi, const float* n, float eta) {
+    return fresnel_exact_norm_free({wi[0], wi[1], wi[2]}, {n[0], n[1], n[2]}, eta);
+}
+// refraction_norm_free (pk.h:181-200); returns 0 on total internal reflection
+int orc_refraction(const float* wi, const float* n, float eta, float* wo) {
+    float3 w;
+    if (!refraction_norm_free(&w, {wi[0], wi[1], wi[2]}, {n[0], n[1], n[2]}, eta)) return 0;
+    wo[0] = w.x, wo[1] = w.y, wo[2] = w.z;
+    return 1;
+}
+// saka::reflection (op 0) / saka::refraction_norm_free (op 1) on one-tangent
+// duals laid out {v.x, v.y, v.z, g.x, g.y, g.z} (saka.h:192-208)
+void orc_dual_optics(int op, const float* wi, const float* n, float eta, float* out) {
+    auto ld = [](const float* p) {
+        dval3<float> r;
+        r.x.v = p[0], r.y.v = p[1], r.z.v = p[2];
+        r.x.g = p[3], r.y.g = p[4], r.z.g = p[5];
+        return r;
+    };
+    const dval3<float> r = op == 0 ? reflection(ld(wi), ld(n)) : refraction_norm_free(ld(wi), ld(n), eta);
+    out[0] = r.x.v, out[1] = r.y.v, out[2] = r.z.v;
+    out[3] = r.x.g, out[4] = r.y.g, out[5] = r.z.g;
+}
+// intersectRayTriangle (minimum_lbvh.h:372-422) on one triangle over
+// [t_min, t_max]; out = {t, u, v, ng.x, ng.y, ng.z} when it returns 1
+int orc_ray_triangle(const palz_triangle* tri, const float* ro, const float* rd, float t_min, float t_max,
+                     float* out) {
+    float t, u, v;
+    float3 ng;
+    if (!intersectRayTriangle(&t, &u, &v, &ng, t_min, t_max, {ro[0], ro[1], ro[2]}, {rd[0], rd[1], rd[2]},
+                              load3(tri->vs[0]), load3(tri->vs[1]), load3(tri->vs[2])))
+        return 0;
+    out[0] = t, out[1] = u, out[2] = v, out[3] = ng.x, out[4] = ng.y, out[5] = ng.z;
+    return 1;
+}
+// intersect_stackfull (minimum_lbvh.h:927-988) for n rays (ro, rd: 3 floats
+// each): hit_t (FLT_MAX on a miss) and hit_tri (0xFFFFFFFF on a miss)
+void orc_closest_hits(const palz_triangle* tris, const palz_internal_node* nodes, uint32_t root, const float* ro,
+                      const float* rd, uint32_t n, float* hit_t, uint32_t* hit_tri) {
+    const Scene s{tris, nullptr, nodes, root};
+    for (uint32_t i = 0; i < n; ++i) {
+        Hit hit;
+        const float3 o = {ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]}, d = {rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]};
+        intersect_stackfull(&hit, s, o, d, invRd(d), nullptr);
+        hit_t[i] = hit.t;
+        hit_tri[i] = hit.tri;
+    }
+}
+// Brute force over every triangle, no BVH: the nearest hit (hit_t, hit_tri as
+// above; on equal t the later triangle wins, as when the stackful walk meets
+// it later) and whether any triangle is hit over [0, t_max] (any_out)
+void orc_brute_hits(const palz_triangle* tris, uint32_t n_tris, const float* ro, const float* rd, uint32_t n,
+                    float t_max, float* hit_t, uint32_t* hit_tri, uint8_t* any_out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float3 o = {ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]}, d = {rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]};
+        float best = 3.402823466e+38F;
+        uint32_t best_tri = 0xFFFFFFFFu;
+        bool any = false;
+        for (uint32_t k = 0; k < n_tris; ++k) {
+            float t, u, v;
+            float3 ng;
+            const palz_triangle& tri = tris[k];
+            if (intersectRayTriangle(&t, &u, &v, &ng, 0.0f, best, o, d, load3(tri.vs[0]), load3(tri.vs[1]),
+                                     load3(tri.vs[2]))) {
+                best = t;
+                best_tri = k;
+                if (t <= t_max) any = true;
+            }
+        }
+        hit_t[i] = best;
+        hit_tri[i] = best_tri;
+        any_out[i] = any ? 1 : 0;
+    }
+}
+// intersect_escape_link (minimum_lbvh.h:1078-1152) over [0, t_max] for n rays
+void orc_any_hits(const palz_triangle* tris, const palz_internal_node* nodes, uint32_t root, const float* ro,
+                  const float* rd, uint32_t n, float t_max, uint8_t* out) {
+    const Scene s{tris, nullptr, nodes, root};
+    for (uint32_t i = 0; i < n; ++i) {
+        const float3 o = {ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]}, d = {rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]};
+        out[i] = any_hit(s, o, d, invRd(d), t_max) ? 1 : 0;
+    }
+}
+// occluded (pk.h:88-115) for n segments (from, from_n, to, to_n: 3 floats each)
+void orc_occluded(const palz_triangle* tris, const palz_internal_node* nodes, uint32_t root, const float* from,
+                  const float* from_n, const float* to, const float* to_n, uint32_t n, uint8_t* out) {
+    const Scene s{tris, nullptr, nodes, root};
+    auto ld = [](const float* p, uint32_t i) { return float3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; };
+    for (uint32_t i = 0; i < n; ++i)
+        out[i] = occluded(s, ld(from, i), ld(from_n, i), ld(to, i), ld(to_n, i)) ? 1 : 0;
+}
+
 }  // extern "C"

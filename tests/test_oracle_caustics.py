@@ -163,15 +163,62 @@ def test_flat_slab_solver_matches_snell_chain():
 
 
 def test_fresnel_reciprocity_reference_case():
-    """unittest.cpp:118-143 through the solver's dielectric throughput: a flat
-    single interface (pool) seen from both sides gives R(wi, n, eta) =
-    R(wo, -n, 1/eta); exercised via the textbook formula agreement above and
-    here directly on the closed form."""
-    rng = np.random.default_rng(103)
+    """unittest.cpp:118-143 with its own PCG(103, 178) inputs, through the
+    oracle's fresnel_exact_norm_free and refraction_norm_free (pk.h:181-200,
+    427-450): for n = (0, 0.1 + 3u, 0) and wi at angle theta in the xy-plane,
+    refraction succeeds and R(wi, n, eta) = R(wo, -n, 1/eta) within 1e-3."""
+    from test_oracle_interval import _PCG
+    f32 = np.float32
+    rng = _PCG(103, 178)
+    pi = f32(3.14159265358979323846)
+    worst = 0.0
     for _ in range(100):
-        eta = 1.0 + rng.random()
-        for i in range(100):
-            th = math.pi * 0.5 * i / 100
-            st = math.sin(th) / eta
-            ct = math.sqrt(1 - st * st)
-            assert abs(_fresnel_unpolarised(math.cos(th), 1.0, eta) - _fresnel_unpolarised(ct, eta, 1.0)) < 1e-3
+        eta = f32(f32(1.0) + rng.uniformf())
+        N = 100
+        for i in range(N):
+            theta = f32(f32(pi * f32(0.5)) * f32(i) / f32(N))
+            n = np.array([0.0, f32(0.1) + rng.uniformf() * f32(3.0), 0.0], np.float32)
+            c, s_ = f32(math.cos(theta)), f32(math.sin(theta))
+            wi = np.array([c * n[0] - s_ * n[1], s_ * n[0] + c * n[1], 0.0], np.float32)
+            R = oracle.fresnel(wi, n, float(eta))
+            wo = oracle.refraction(wi, n, float(eta))
+            assert wo is not None
+            R_inv = oracle.fresnel(wo, -n, float(f32(1.0) / eta))
+            worst = max(worst, abs(R - R_inv))
+    assert worst < 1e-3
+
+
+def test_fresnel_and_refraction_match_the_textbook_forms():
+    """The norm-free forms on unnormalised vectors against float64 Snell and
+    the s/p-averaged Fresnel equations on the normalised ones, both sides of
+    the interface, total internal reflection included."""
+    rng = np.random.default_rng(17)
+    n_tir = 0
+    for _ in range(3000):
+        n = rng.normal(size=3)
+        wi = rng.normal(size=3)
+        eta = rng.uniform(0.6, 1.9)
+        nh, wh = n / np.linalg.norm(n), wi / np.linalg.norm(wi)
+        ci = float(nh @ wh)
+        if abs(ci) < 1e-3:
+            continue
+        n1, n2 = (1.0, eta) if ci > 0 else (eta, 1.0)
+        want_R = _fresnel_unpolarised(abs(ci), n1, n2)
+        got_R = oracle.fresnel(wi.astype(np.float32), n.astype(np.float32), float(np.float32(eta)))
+        assert abs(got_R - want_R) < 2e-4, (wi, n, eta)
+        wo = oracle.refraction(wi.astype(np.float32), n.astype(np.float32), float(np.float32(eta)))
+        sin_t2 = (n1 / n2) ** 2 * (1.0 - ci * ci)
+        if sin_t2 > 1.0 + 1e-4:
+            assert wo is None and got_R == 1.0
+            n_tir += 1
+            continue
+        if sin_t2 > 1.0 - 1e-4:
+            continue
+        # Snell: transmitted direction on the far side of the interface
+        ns = nh if ci > 0 else -nh
+        ct = math.sqrt(1.0 - sin_t2)
+        t = -wh * (n1 / n2) + ns * ((n1 / n2) * abs(ci) - ct)
+        assert wo is not None
+        woh = wo.astype(np.float64) / np.linalg.norm(wo)
+        assert np.linalg.norm(woh - t) < 1e-5, (wi, n, eta)
+    assert n_tir > 100

@@ -1,0 +1,148 @@
+"""Pins of the oracle's BVH traversals and ray primitive (CPU only).
+
+  * intersect_stackfull (minimum_lbvh.h:927-988): the nearest hit equals a
+    brute force over every triangle (no BVH), t bit for bit, on random soups
+    with heavy overlap, on the reference's table mesh and on a caustic scene.
+  * intersect_escape_link (minimum_lbvh.h:1078-1152): any-hit over [0, 1]
+    equals the brute-force segment test on the same rays.  Both outcomes
+    occur in every case, so a walk that never (or always) reports a hit fails.
+  * occluded (pk.h:88-115) on a scene whose answers are fixed by geometry: a
+    diffuse occluder square between two parallel planes blocks exactly the
+    segments whose crossing point lies inside it; segments that start and end
+    on the planes themselves are not blocked by those planes (the offsets).
+  * intersectRayTriangle (minimum_lbvh.h:372-422) against a float64
+    Moller-Trumbore intersection away from the triangle edges.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from palzilla_amd import scenes
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _bvh(tris):
+    nodes, root, *_ = oracle.lbvh(tris, bits=32)
+    return nodes, root
+
+
+def _rays_into(tris, n, rng, spread=1.2):
+    lo = tris.reshape(-1, 3).min(axis=0)
+    hi = tris.reshape(-1, 3).max(axis=0)
+    c = 0.5 * (lo + hi)
+    ext = np.max(hi - lo)
+    ro = c + rng.normal(size=(n, 3)) * ext
+    target = c + (rng.random((n, 3)) - 0.5) * ext * spread
+    rd = target - ro
+    return ro.astype(np.float32), rd.astype(np.float32)
+
+
+def _cases():
+    rng = np.random.default_rng(11)
+    soup = scenes.triangle_soup(3000, seed=5, mu=-2.0, sigma=0.4)
+    table = scenes.load_obj(os.path.join(GOLD, "table.obj"))
+    slab = scenes.config2(cells=16, width=8, photons=1 << 10, n_lambdas=1).tris
+    return {"soup": (soup, rng), "table": (table, rng), "slab": (slab, rng)}
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_closest_hit_equals_brute_force(name):
+    tris, rng = CASES[name]
+    nodes, root = _bvh(tris)
+    ro, rd = _rays_into(tris, 1500, rng)
+    t, tri = oracle.closest_hits(tris, nodes, root, ro, rd)
+    bt, btri, _ = oracle.brute_hits(tris, ro, rd)
+    hit = bt < np.float32(3.4e38)
+    assert 0.2 < hit.mean() < 0.98, "both hits and misses must occur"
+    assert np.array_equal(t.view(np.uint32), bt.view(np.uint32)), "nearest t differs from the brute force"
+    diff = np.nonzero(tri != btri)[0]
+    # only exact ties in t may pick another triangle; that triangle must be hit at the same t
+    for i in diff:
+        r = oracle.ray_triangle(tris[tri[i]], ro[i], rd[i])
+        assert r is not None and r[0] == bt[i], f"ray {i}: triangle {tri[i]} is not a nearest hit"
+    assert len(diff) <= len(tri) // 100
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_any_hit_equals_brute_force_segment_test(name):
+    tris, rng = CASES[name]
+    nodes, root = _bvh(tris)
+    ro, rd = _rays_into(tris, 1500, rng)
+    # segments of random length along the rays
+    rd = rd * rng.uniform(0.3, 1.3, size=(len(rd), 1)).astype(np.float32)
+    got = oracle.any_hits(tris, nodes, root, ro, rd, 1.0)
+    _, _, want = oracle.brute_hits(tris, ro, rd, 1.0)
+    assert 0.1 < want.mean() < 0.95, "both blocked and free segments must occur"
+    assert np.array_equal(got, want)
+
+
+def _occluder_scene():
+    """Floor quad (y = 0), ceiling quad (y = 1) and an occluder square
+    [-0.25, 0.25]^2 at y = 0.5, all two triangles each."""
+    def quad(y, h):
+        a, b, c, d = [(-h, y, -h), (h, y, -h), (h, y, h), (-h, y, h)]
+        return [a + b + c, a + c + d]
+    tris = np.array(quad(0.0, 2.0) + quad(1.0, 2.0) + quad(0.5, 0.25), np.float32)
+    return tris
+
+
+def test_occluded_matches_the_occluder_geometry():
+    tris = _occluder_scene()
+    nodes, root = _bvh(tris)
+    rng = np.random.default_rng(3)
+    n = 4000
+    p0 = np.column_stack([rng.uniform(-1, 1, n), np.zeros(n), rng.uniform(-1, 1, n)]).astype(np.float32)
+    p1 = np.column_stack([rng.uniform(-1, 1, n), np.ones(n), rng.uniform(-1, 1, n)]).astype(np.float32)
+    up = np.tile(np.float32([0, 1, 0]), (n, 1))
+    got = oracle.occluded(tris, nodes, root, p0, up, p1, -up)
+    mid = 0.5 * (p0.astype(np.float64) + p1)  # where the segment crosses y = 0.5
+    inside = np.all(np.abs(mid[:, [0, 2]]) < 0.25 - 1e-4, axis=1)
+    outside = np.any(np.abs(mid[:, [0, 2]]) > 0.25 + 1e-4, axis=1)
+    assert inside.sum() > 100 and outside.sum() > 1000
+    assert got[inside].all(), "a segment through the occluder was not blocked"
+    assert not got[outside].any(), "a free segment was blocked (the end planes or a miss)"
+    # the end points' own planes never block: normals given either way round
+    got_flipped = oracle.occluded(tris, nodes, root, p0, -up, p1, up)
+    assert np.array_equal(got, got_flipped)
+
+
+def test_ray_triangle_agrees_with_float64_moller_trumbore():
+    rng = np.random.default_rng(5)
+    tris = scenes.triangle_soup(400, seed=9, mu=-1.0, sigma=0.3)
+    n_checked = n_hit = 0
+    for k in range(400):
+        v = tris[k].reshape(3, 3).astype(np.float64)
+        for _ in range(10):
+            ro = rng.normal(size=3) * 2.0
+            # aim at a random point of the triangle's plane near it
+            w = rng.uniform(-0.3, 1.3, size=2)
+            target = v[0] + (v[1] - v[0]) * w[0] + (v[2] - v[0]) * w[1]
+            rd = target - ro
+            e1, e2 = v[1] - v[0], v[2] - v[0]
+            pv = np.cross(rd, e2)
+            det = e1 @ pv
+            tv = ro - v[0]
+            u = (tv @ pv) / det
+            qv = np.cross(tv, e1)
+            vv = (rd @ qv) / det
+            t = (e2 @ qv) / det
+            margin = min(u, vv, 1.0 - u - vv)
+            grazing = abs(det) / (np.linalg.norm(rd) * np.linalg.norm(np.cross(e1, e2))) < 0.05
+            if abs(margin) < 1e-3 or grazing or t < 1e-3:
+                continue
+            got = oracle.ray_triangle(tris[k], ro.astype(np.float32), rd.astype(np.float32))
+            n_checked += 1
+            assert (got is not None) == (margin > 0.0)
+            if got is not None:
+                n_hit += 1
+                assert abs(got[0] - t) <= 1e-5 * max(1.0, abs(t))
+                # (u, v) are the weights of v1 and v2: each edge volume is the
+                # one of the edge opposite its vertex (minimum_lbvh.h:383-397)
+                assert abs(got[1] - u) < 1e-4 and abs(got[2] - vv) < 1e-4
+    assert n_checked > 2000 and 0.1 * n_checked < n_hit < 0.9 * n_checked
