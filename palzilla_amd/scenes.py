@@ -186,6 +186,27 @@ def stack_scene(*, mirror_cells=0, layer="none", layer_cells=0, ior=1.5, cauchy_
                               "ior": ior, "photons": photons, "seed": seed})
 
 
+def with_occluders(scene: CausticScene, squares) -> CausticScene:
+    """The scene with diffuse occluder squares appended after its triangles
+    (so every existing triangle keeps its index and a path list made for the
+    scene stays valid).  squares: (y, x0, x1, z0, z1) per horizontal square,
+    two triangles each, shading normals +y."""
+    parts = []
+    for y, x0, x1, z0, z1 in squares:
+        a, b, c, d = (x0, y, z0), (x1, y, z0), (x1, y, z1), (x0, y, z1)
+        parts.append([a + b + c, a + c + d])
+    extra = np.asarray(parts, np.float64).reshape(-1, 9).astype(np.float32)
+    att = np.zeros(extra.shape[0], dtype=T.TRIANGLE_ATTRIB)
+    att["material"] = T.MATERIAL_DIFFUSE
+    att["shading_normals"] = np.tile(np.float32([0.0, 1.0, 0.0]), (extra.shape[0], 3, 1))
+    info = dict(scene.info, occluders=[tuple(q) for q in squares])
+    return CausticScene(tris=np.concatenate([scene.tris, extra]), attribs=np.concatenate([scene.attribs, att]),
+                        p_light=scene.p_light, points=scene.points, width=scene.width, height=scene.height,
+                        extent=scene.extent, cauchy_a=scene.cauchy_a, cauchy_b=scene.cauchy_b, lambdas=scene.lambdas,
+                        photons_per_triangle=scene.photons_per_triangle, eta_min=scene.eta_min,
+                        eta_max=scene.eta_max, name=scene.name + "+occluders", info=info)
+
+
 def config1(seed: int = 1, width: int = 256, photons: int = 1 << 16) -> CausticScene:
     """Single-bounce mirror caustic: 64x64-cell mirror heightfield (8,192
     triangles) over [-1,1]^2 at y = 0.5, 4 sinusoids of amplitude 0.02,
