@@ -137,6 +137,15 @@ int palz_sorter_reserve(palz_sorter* s, uint32_t n);
 int palz_sort_u64(palz_sorter* s, uint64_t* words, uint64_t* tmp, uint32_t* values,
                   uint32_t* tmp_values, uint32_t n, int start_bit, int end_bit, void* stream);
 
+/* Stable LSD sort of 32-bit keys on bit range [start_bit, end_bit) (end_bit
+ * <= 32), 8-bit digits, with an optional u32 payload.  Replaces
+ * OnesweepSort::sort(const KeyValueSoA32&, const KeyValueSoA32&, n, startBit,
+ * endBit, stream) (tinyhiponesweep.h:562-566, :592-595): `keys`/`values` are
+ * KeyValueSoA32's key/value arrays, `tmp`/`tmp_values` its tmp's.  The result
+ * is written back to `keys` (and `values`).  n must be < 2^30. */
+int palz_sort_u32(palz_sorter* s, uint32_t* keys, uint32_t* tmp, uint32_t* values, uint32_t* tmp_values,
+                  uint32_t n, int start_bit, int end_bit, void* stream);
+
 /* ---- LBVH builder ---------------------------------------------------------
  * Replaces minimum_lbvh::BVHGPUBuilder (minimum_lbvh.h:741-900).  The builder
  * owns the node array (n-1 records) and the root index, as the reference does

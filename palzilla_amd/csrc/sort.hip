@@ -2,7 +2,8 @@
 //
 // Semantics follow tinyhiponesweep::OnesweepSort::sort (tinyhiponesweep.h:
 // 592-668): a stable LSD sort with 8-bit digits over [startBit, endBit) of
-// 64-bit words, optional u32 payload.  The design is MI355X-native:
+// 64-bit words or 32-bit keys, optional u32 payload.  The design is
+// MI355X-native:
 //   * one digit count pass reads the words once for every digit position
 //     (LDS histograms, one global atomic per non-empty bin and block);
 //   * each digit pass is ONE kernel: a 4096-key tile per 256-lane block, ranked
@@ -12,25 +13,24 @@
 //   * tiles are handed out by a ticket counter, so a block only ever waits on
 //     tiles owned by blocks that started before it;
 //   * look-back words carry an epoch that a device-side counter bumps at the
-//     start of every sort, so the table is never cleared and the sort can be
-//     captured in a hipGraph and replayed.
+//     start of every sort, so the table is never cleared between sorts and the
+//     sort can be captured in a hipGraph and replayed (the word layout is in
+//     lookback.hpp; a sorter clears its table once per 2^27 uses, long before
+//     a tag could repeat).
 #include "sort.hpp"
 
 #include <algorithm>
+
+#include "lookback.hpp"
 
 namespace palz {
 
 namespace {
 
-constexpr uint32_t kFlagAggregate = 1u;
-constexpr uint32_t kFlagInclusive = 2u;
 constexpr int kLookbackWindow = 8;  // predecessor tiles read per look-back step
 
-__device__ __forceinline__ uint64_t lb_pack(uint32_t ep, uint32_t flag, uint32_t count) {
-    return ((uint64_t)((ep << 2) | flag) << 32) | (uint64_t)count;
-}
-
-__device__ __forceinline__ uint32_t digit_of(uint64_t w, int shift, uint32_t mask) {
+template <class KeyT>
+__device__ __forceinline__ uint32_t digit_of(KeyT w, int shift, uint32_t mask) {
     return (uint32_t)(w >> shift) & mask;
 }
 
@@ -65,7 +65,16 @@ __global__ void __launch_bounds__(256) k_sort_prepare(uint32_t* hist, uint32_t* 
     }
 }
 
-__global__ void __launch_bounds__(256) k_sort_count(const uint64_t* __restrict__ words, uint32_t n,
+// A fresh look-back table: every word zero (tag 0, which no sort uses) and
+// the epoch back at 0.
+__global__ void __launch_bounds__(256) k_sort_clear(uint64_t* __restrict__ lookback, size_t n_words,
+                                                    uint32_t* __restrict__ epoch) {
+    for (size_t i = blockIdx.x * 256u + threadIdx.x; i < n_words; i += (size_t)gridDim.x * 256u) lookback[i] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) epoch[0] = 0u;
+}
+
+template <class KeyT>
+__global__ void __launch_bounds__(256) k_sort_count(const KeyT* __restrict__ words, uint32_t n,
                                                     int start_bit, int end_bit, int npass,
                                                     uint32_t* __restrict__ hist,
                                                     const uint32_t* __restrict__ n_dev) {
@@ -75,7 +84,7 @@ __global__ void __launch_bounds__(256) k_sort_count(const uint64_t* __restrict__
     __syncthreads();
     const uint32_t stride = gridDim.x * 256u;
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += stride) {
-        const uint64_t w = words[i];
+        const KeyT w = words[i];
         for (int p = 0; p < npass; ++p) {
             const int shift = start_bit + 8 * p;
             const int bits = min(8, end_bit - shift);
@@ -89,9 +98,9 @@ __global__ void __launch_bounds__(256) k_sort_count(const uint64_t* __restrict__
     }
 }
 
-template <bool kValues>
+template <class KeyT, bool kValues>
 __global__ void __launch_bounds__(kSortThreads)
-    k_sort_pass(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
+    k_sort_pass(const KeyT* __restrict__ src, KeyT* __restrict__ dst,
                 const uint32_t* __restrict__ vsrc, uint32_t* __restrict__ vdst, uint32_t n,
                 int shift, uint32_t mask, int pass, const uint32_t* __restrict__ hist,
                 uint64_t* __restrict__ lookback, uint32_t* __restrict__ ticket,
@@ -102,7 +111,7 @@ __global__ void __launch_bounds__(kSortThreads)
     __shared__ int32_t s_scatter_base[256];
     __shared__ uint32_t s_tot[kSortWaves];
     __shared__ uint32_t s_tile;
-    __shared__ uint64_t s_keys[kSortTile];
+    __shared__ KeyT s_keys[kSortTile];
     __shared__ uint32_t s_vals[kValues ? kSortTile : 1];
 
     const int tid = threadIdx.x;
@@ -120,18 +129,18 @@ __global__ void __launch_bounds__(kSortThreads)
         n = min(n, *n_dev);  // blocks past the device-side count have nothing to do
         if (tile * (uint32_t)kSortTile >= n) return;
     }
-    const uint32_t ep = epoch[0] * 8u + (uint32_t)pass;
+    const uint32_t ep = lb_tag(epoch[0], (uint32_t)pass);
     const uint32_t tile_base = tile * (uint32_t)kSortTile;
     const uint32_t wave_base = tile_base + (uint32_t)wave * (64u * kSortItems);
 
-    uint64_t key[kSortItems];
+    KeyT key[kSortItems];
     uint32_t val[kValues ? kSortItems : 1];
     uint32_t rank[kSortItems];
 
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
         const uint32_t idx = wave_base + (uint32_t)j * 64u + (uint32_t)lane;
-        key[j] = idx < n ? src[idx] : 0ull;
+        key[j] = idx < n ? src[idx] : (KeyT)0;
         if constexpr (kValues) val[j] = idx < n ? vsrc[idx] : 0u;
     }
 
@@ -146,7 +155,7 @@ __global__ void __launch_bounds__(kSortThreads)
     const uint32_t d = (uint32_t)tid;  // one digit per thread from here
     const uint32_t count = s_count[d];
     uint64_t* my = lookback + (size_t)tile * 256u + d;
-    __hip_atomic_store(my, lb_pack(ep, tile == 0 ? kFlagInclusive : kFlagAggregate, count), __ATOMIC_RELAXED,
+    __hip_atomic_store(my, lb_pack(ep, tile == 0 ? kLbFlagInclusive : kLbFlagAggregate, count), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
 
     // Wave64 multisplit: rank each key among equal digits, in key order.
@@ -199,26 +208,25 @@ __global__ void __launch_bounds__(kSortThreads)
             for (int w = 0; w < kLookbackWindow; ++w)
                 e[w] = p - w >= 0 ? __hip_atomic_load(lookback + (size_t)(p - w) * 256u + d, __ATOMIC_RELAXED,
                                                       __HIP_MEMORY_SCOPE_AGENT)
-                                  : lb_pack(ep, kFlagInclusive, 0u);
+                                  : lb_pack(ep, kLbFlagInclusive, 0u);
             int used = 0;
             bool done = false, stalled = false;
 #pragma unroll
             for (int w = 0; w < kLookbackWindow; ++w) {
                 if (done || stalled) continue;
-                const uint32_t hi = (uint32_t)(e[w] >> 32);
-                if ((hi >> 2) != ep) {
+                if (!lb_current(e[w], ep)) {
                     stalled = true;
                     continue;
                 }
-                excl += (uint32_t)e[w];
+                excl += lb_count(e[w]);
                 ++used;
-                done = (hi & 3u) == kFlagInclusive;
+                done = lb_flag(e[w]) == kLbFlagInclusive;
             }
             if (done) break;
             p -= used;
             if (stalled) __builtin_amdgcn_s_sleep(1);
         }
-        __hip_atomic_store(my, lb_pack(ep, kFlagInclusive, excl + count), __ATOMIC_RELAXED,
+        __hip_atomic_store(my, lb_pack(ep, kLbFlagInclusive, excl + count), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
 
@@ -244,12 +252,48 @@ __global__ void __launch_bounds__(kSortThreads)
     // Digit runs are contiguous in LDS and in the output: coalesced scatter.
     const uint32_t tile_n = min((uint32_t)kSortTile, n - tile_base);
     for (uint32_t i = (uint32_t)tid; i < tile_n; i += kSortThreads) {
-        const uint64_t k = s_keys[i];
+        const KeyT k = s_keys[i];
         const uint32_t di = digit_of(k, shift, mask);
         const uint32_t g = (uint32_t)(s_scatter_base[di] + (int32_t)i);
         dst[g] = k;
         if constexpr (kValues) vdst[g] = s_vals[i];
     }
+}
+
+template <class KeyT>
+void sort_count(const SortScratch& s, const KeyT* words, uint32_t n, int start_bit, int end_bit,
+                hipStream_t stream, const uint32_t* n_dev) {
+    const int npass = (int)div_up((uint32_t)(end_bit - start_bit), 8u);
+    const uint32_t blocks = std::max(1u, std::min(div_up(n, 256u * 16u), 2048u));
+    k_sort_count<KeyT><<<blocks, 256, 0, stream>>>(words, n, start_bit, end_bit, npass, s.hist, n_dev);
+}
+
+template <class KeyT>
+int sort_passes(const SortScratch& s, KeyT* words, KeyT* tmp, uint32_t* values, uint32_t* tmp_values, uint32_t n,
+                int start_bit, int end_bit, hipStream_t stream, hipEvent_t* pass_events, const uint32_t* n_dev) {
+    const int npass = (int)div_up((uint32_t)(end_bit - start_bit), 8u);
+    const uint32_t tiles = div_up(n, (uint32_t)kSortTile);
+    KeyT* src = words;
+    KeyT* dst = tmp;
+    uint32_t* vsrc = values;
+    uint32_t* vdst = tmp_values;
+    for (int p = 0; p < npass; ++p) {
+        const int shift = start_bit + 8 * p;
+        const int bits = std::min(8, end_bit - shift);
+        const uint32_t mask = (1u << bits) - 1u;
+        if (values) {
+            k_sort_pass<KeyT, true><<<tiles, kSortThreads, 0, stream>>>(
+                src, dst, vsrc, vdst, n, shift, mask, p, s.hist + p * 256, s.lookback, s.tickets + p, s.epoch, n_dev);
+        } else {
+            k_sort_pass<KeyT, false><<<tiles, kSortThreads, 0, stream>>>(
+                src, dst, nullptr, nullptr, n, shift, mask, p, s.hist + p * 256, s.lookback, s.tickets + p, s.epoch,
+                n_dev);
+        }
+        if (pass_events) (void)hipEventRecord(pass_events[p], stream);
+        std::swap(src, dst);
+        std::swap(vsrc, vdst);
+    }
+    return npass;
 }
 
 }  // namespace
@@ -258,40 +302,39 @@ void launch_sort_prepare(const SortScratch& s, palz_aabb* aabb, hipStream_t stre
     k_sort_prepare<<<1, 256, 0, stream>>>(s.hist, s.tickets, s.epoch, aabb);
 }
 
+void launch_sort_clear(const SortScratch& s, hipStream_t stream) {
+    const size_t words = (size_t)s.lookback_tiles * 256u;
+    const uint32_t blocks = (uint32_t)std::max<size_t>(1, std::min<size_t>((words + 255u) / 256u, 4096));
+    k_sort_clear<<<blocks, 256, 0, stream>>>(s.lookback, words, s.epoch);
+}
+
 void launch_sort_count(const SortScratch& s, const uint64_t* words, uint32_t n, int start_bit,
                        int end_bit, hipStream_t stream, const uint32_t* n_dev) {
-    const int npass = (int)div_up((uint32_t)(end_bit - start_bit), 8u);
-    const uint32_t blocks = std::max(1u, std::min(div_up(n, 256u * 16u), 2048u));
-    k_sort_count<<<blocks, 256, 0, stream>>>(words, n, start_bit, end_bit, npass, s.hist, n_dev);
+    sort_count<uint64_t>(s, words, n, start_bit, end_bit, stream, n_dev);
 }
 
 int launch_sort_passes(const SortScratch& s, uint64_t* words, uint64_t* tmp, uint32_t* values,
                        uint32_t* tmp_values, uint32_t n, int start_bit, int end_bit,
                        hipStream_t stream, hipEvent_t* pass_events, const uint32_t* n_dev) {
-    const int npass = (int)div_up((uint32_t)(end_bit - start_bit), 8u);
-    const uint32_t tiles = div_up(n, (uint32_t)kSortTile);
-    uint64_t* src = words;
-    uint64_t* dst = tmp;
-    uint32_t* vsrc = values;
-    uint32_t* vdst = tmp_values;
-    for (int p = 0; p < npass; ++p) {
-        const int shift = start_bit + 8 * p;
-        const int bits = std::min(8, end_bit - shift);
-        const uint32_t mask = (1u << bits) - 1u;
-        if (values) {
-            k_sort_pass<true><<<tiles, kSortThreads, 0, stream>>>(
-                src, dst, vsrc, vdst, n, shift, mask, p, s.hist + p * 256, s.lookback,
-                s.tickets + p, s.epoch, n_dev);
-        } else {
-            k_sort_pass<false><<<tiles, kSortThreads, 0, stream>>>(
-                src, dst, nullptr, nullptr, n, shift, mask, p, s.hist + p * 256, s.lookback,
-                s.tickets + p, s.epoch, n_dev);
-        }
-        if (pass_events) (void)hipEventRecord(pass_events[p], stream);
-        std::swap(src, dst);
-        std::swap(vsrc, vdst);
+    return sort_passes<uint64_t>(s, words, tmp, values, tmp_values, n, start_bit, end_bit, stream, pass_events,
+                                 n_dev);
+}
+
+// A whole sort on the stream: prepare, count, passes, and the copy back when
+// the number of passes is odd.  Sizes and bits are checked by the callers.
+template <class KeyT>
+int sort_all(const SortScratch& s, KeyT* words, KeyT* tmp, uint32_t* values, uint32_t* tmp_values, uint32_t n,
+             int start_bit, int end_bit, hipStream_t st) {
+    launch_sort_prepare(s, nullptr, st);
+    sort_count<KeyT>(s, words, n, start_bit, end_bit, st, nullptr);
+    const int npass = sort_passes<KeyT>(s, words, tmp, values, tmp_values, n, start_bit, end_bit, st, nullptr,
+                                        nullptr);
+    PALZ_LAUNCH_CHECK();
+    if (npass & 1) {
+        PALZ_HIP(hipMemcpyAsync(words, tmp, sizeof(KeyT) * n, hipMemcpyDeviceToDevice, st));
+        if (values) PALZ_HIP(hipMemcpyAsync(values, tmp_values, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
     }
-    return npass;
+    return PALZ_OK;
 }
 
 }  // namespace palz
@@ -303,6 +346,14 @@ int palz_sorter::reserve(uint32_t n) {
         PALZ_HIP(hipMalloc(&scratch.tickets, sizeof(uint32_t) * kSortMaxPasses));
         PALZ_HIP(hipMalloc(&scratch.epoch, sizeof(uint32_t)));
         PALZ_HIP(hipMemset(scratch.epoch, 0, sizeof(uint32_t)));
+    }
+    // Each use runs at most two sorts; the table is cleared long before the
+    // tags epoch * 8 + pass could come round again.
+    if (++uses >= kLbSortsPerClear / 2 && scratch.lookback) {
+        PALZ_HIP(hipDeviceSynchronize());
+        launch_sort_clear(scratch, nullptr);
+        PALZ_HIP(hipDeviceSynchronize());
+        uses = 0;
     }
     const uint32_t tiles = std::max(1u, div_up(n, (uint32_t)kSortTile));
     if (tiles > scratch.lookback_tiles) {
@@ -364,19 +415,25 @@ int palz_sort_u64(palz_sorter* s, uint64_t* words, uint64_t* tmp, uint32_t* valu
         return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_sort_u64: values and tmp_values go together");
     const int rc = s->reserve(n);
     if (rc != PALZ_OK) return rc;
-    hipStream_t st = (hipStream_t)stream;
-    launch_sort_prepare(s->scratch, nullptr, st);
-    launch_sort_count(s->scratch, words, n, start_bit, end_bit, st);
-    const int npass = launch_sort_passes(s->scratch, words, tmp, values, tmp_values, n, start_bit,
-                                         end_bit, st);
-    PALZ_LAUNCH_CHECK();
-    if (npass & 1) {
-        PALZ_HIP(hipMemcpyAsync(words, tmp, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, st));
-        if (values)
-            PALZ_HIP(hipMemcpyAsync(values, tmp_values, sizeof(uint32_t) * n,
-                                    hipMemcpyDeviceToDevice, st));
-    }
-    return PALZ_OK;
+    return sort_all<uint64_t>(s->scratch, words, tmp, values, tmp_values, n, start_bit, end_bit,
+                              (hipStream_t)stream);
+}
+
+int palz_sort_u32(palz_sorter* s, uint32_t* keys, uint32_t* tmp, uint32_t* values, uint32_t* tmp_values,
+                  uint32_t n, int start_bit, int end_bit, void* stream) {
+    using namespace palz;
+    if (!s) return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_sort_u32: sorter is NULL");
+    if (start_bit < 0 || end_bit > 32 || start_bit >= end_bit)
+        return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_sort_u32: need 0 <= start_bit < end_bit <= 32");
+    if (n > kSortMaxN) return fail(PALZ_ERR_CAPACITY, "palz_sort_u32: n must be < 2^30");
+    if (n == 0) return PALZ_OK;
+    if (!keys || !tmp) return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_sort_u32: keys/tmp is NULL");
+    if ((values == nullptr) != (tmp_values == nullptr))
+        return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_sort_u32: values and tmp_values go together");
+    const int rc = s->reserve(n);
+    if (rc != PALZ_OK) return rc;
+    return sort_all<uint32_t>(s->scratch, keys, tmp, values, tmp_values, n, start_bit, end_bit,
+                              (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -12,7 +12,7 @@ constexpr int kSortWaves = kSortThreads / 64;
 constexpr int kSortItems = 16;                     // keys per lane
 constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
 constexpr int kSortMaxPasses = 8;                  // 64-bit words, 8-bit digits
-constexpr uint32_t kSortMaxN = (1u << 30) - 1;     // look-back counts are 30-bit
+constexpr uint32_t kSortMaxN = (1u << 30) - 1;     // look-back counts are 30-bit (lookback.hpp)
 
 struct SortScratch {
     uint32_t* hist = nullptr;      // [kSortMaxPasses][256] digit histograms
@@ -21,6 +21,9 @@ struct SortScratch {
     uint64_t* lookback = nullptr;  // [tiles * 256] (epoch, flag, count) words
     uint32_t lookback_tiles = 0;
 };
+
+// Clear a look-back table: every word zero and the epoch 0 (lookback.hpp).
+void launch_sort_clear(const SortScratch& s, hipStream_t stream);
 
 // Zero the histograms and tile tickets, bump the epoch.  If `aabb` is not null
 // it is reset to the empty box (the LBVH build uses the same launch).
@@ -43,6 +46,7 @@ int launch_sort_passes(const SortScratch& s, uint64_t* words, uint64_t* tmp, uin
 
 struct palz_sorter {
     palz::SortScratch scratch;
+    uint64_t uses = 0;        // reserve() calls since the look-back table was last cleared
     int reserve(uint32_t n);  // grows the look-back table; allocates, so not graph-safe
     ~palz_sorter();
 };

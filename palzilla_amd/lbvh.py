@@ -34,6 +34,14 @@ class OnesweepSort:
              C.c_void_p(values or None), C.c_void_p(tmp_values or None), C.c_uint32(n),
              int(start_bit), int(end_bit), _sp(stream))
 
+    def sort32(self, keys: int, tmp: int, n: int, start_bit: int, end_bit: int, stream=None,
+               values: int = 0, tmp_values: int = 0) -> None:
+        """OnesweepSort::sort(KeyValueSoA32{keys, values}, KeyValueSoA32{tmp, tmp_values},
+        n, startBit, endBit, stream): stable LSD sort of device u32 keys in place."""
+        call("palz_sort_u32", C.c_void_p(self.handle), C.c_void_p(keys), C.c_void_p(tmp),
+             C.c_void_p(values or None), C.c_void_p(tmp_values or None), C.c_uint32(n),
+             int(start_bit), int(end_bit), _sp(stream))
+
     def __del__(self):
         try:
             if self.handle:

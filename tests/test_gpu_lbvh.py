@@ -49,6 +49,27 @@ def test_sort_matches_stable_argsort(sorter, n, start, end):
     assert np.array_equal(got_only, w[order])
 
 
+@pytest.mark.parametrize("n", [1, 4097, 65539, 1 << 20])
+@pytest.mark.parametrize("start,end", [(0, 32), (0, 30), (5, 21)])
+def test_sort_u32_matches_stable_argsort(sorter, n, start, end):
+    # the KeyValueSoA32 overload (tinyhiponesweep.h:592): u32 keys, optional u32 values
+    print(f"sort32 n={n} bits=[{start},{end})", flush=True)
+    rng = np.random.default_rng(3 * n + start + end)
+    k = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32)
+    if n > 10:
+        k[::7] = k[2]
+    v = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32)
+    order = np.argsort((k >> np.uint32(start)) & np.uint32((1 << (end - start)) - 1), kind="stable")
+    dk, dt = DeviceBuffer.from_array(k), DeviceBuffer(4 * n)
+    dv, dtv = DeviceBuffer.from_array(v), DeviceBuffer(4 * n)
+    sorter.sort32(dk.ptr, dt.ptr, n, start, end, values=dv.ptr, tmp_values=dtv.ptr)
+    assert np.array_equal(dk.download(np.uint32, n), k[order])
+    assert np.array_equal(dv.download(np.uint32, n), v[order])
+    dk2 = DeviceBuffer.from_array(k)
+    sorter.sort32(dk2.ptr, dt.ptr, n, start, end)
+    assert np.array_equal(dk2.download(np.uint32, n), k[order])
+
+
 def test_sort_skewed_digits(sorter):
     # all keys share the high digits: one bin holds the whole tile in late passes
     n = 300_000
