@@ -87,30 +87,34 @@ def measure_copy_bandwidth(nbytes: int = 1 << 30, reps: int = 5) -> float:
 # ---------------------------------------------------------------------------------------------------
 # spectral slab (headline)
 
-def NEWTON_BYTES(K: int) -> int:
-    """Compulsory HBM bytes per (path, wavelength) item of k_newton<K>: the
-    path record (64), its K triangles (36 each) and attributes (40 each), the
-    receiver point (40) and wavelength (4); written: the converged flag (4),
-    solution (8K) and iteration count (4), or a 48-byte result record
-    (the larger of the two, 48, is counted)."""
-    return 64 + 76 * K + 40 + 4 + 48
+def NEWTON_BYTES(K: int) -> tuple[int, int]:
+    """Compulsory HBM bytes of k_newton<K> as (per path, per item).  Per path
+    (its L wavelength items sit next to each other and share these): the path
+    record (64), its K triangles (36 each) and attributes (40 each) and the
+    receiver point (40).  Per (path, wavelength) item: the wavelength (4) and,
+    written, the converged flag (4) and either the solution (8K) and iteration
+    count (4) or a 48-byte result record (the larger, 48, is counted)."""
+    return 64 + 76 * K + 40, 4 + 4 + 48
 
 
-def OCCLUDE_BYTES(K: int) -> int:
-    """Compulsory HBM bytes per converged item of k_occlude<K>: the list entry
-    (4), path record (64), FP32 solution (8K), K triangles and attributes (76
-    each), receiver point (40) and wavelength (4); written: the throughput (4).
-    BVH nodes and triangles read by the occlusion rays are not counted (the
-    BVH stays in cache)."""
-    return 4 + 64 + 8 * K + 76 * K + 40 + 4 + 4
+def OCCLUDE_BYTES(K: int) -> tuple[int, int]:
+    """Compulsory HBM bytes of k_occlude<K> as (per path with a converged
+    item, per converged item).  Per path: the path record (64), K triangles
+    and attributes (76 each) and the receiver point (40).  Per item: the list
+    entry (4), FP32 solution (8K), wavelength (4) and, written, the throughput
+    (4).  BVH nodes and triangles read by the occlusion rays are not counted
+    (the BVH stays in cache): the kernel's rate is given by its rays and node
+    visits as well."""
+    return 64 + 76 * K + 40, 4 + 8 * K + 4 + 4
 
 
-def FINISH_BYTES(K: int) -> int:
-    """Compulsory HBM bytes per converged item of k_finish<K>: the list entry
-    (4), path record (64), FP32 solution (8K), iteration count (4) and
-    throughput (4), K triangles and attributes (76 each), receiver point (40),
-    wavelength (4), the 48-byte result record and the image word (4)."""
-    return 4 + 64 + 8 * K + 4 + 4 + 76 * K + 40 + 4 + 48 + 4
+def FINISH_BYTES(K: int) -> tuple[int, int]:
+    """Compulsory HBM bytes of k_finish<K> as (per path with a converged item,
+    per converged item).  Per path: path record (64), K triangles and
+    attributes (76 each), receiver point (40).  Per item: list entry (4), FP32
+    solution (8K), iteration count (4), throughput (4), wavelength (4), the
+    48-byte result record and the image word (4)."""
+    return 64 + 76 * K + 40, 4 + 8 * K + 4 + 4 + 4 + 48 + 4
 
 
 def profiled_traffic(workload: str, kernel: str):
@@ -227,6 +231,7 @@ def bench_c2(args) -> dict:
     bs.set_timing(False)
     ms = t0.elapsed_ms(t1) / args.steps
     stats = bs.stats()
+    path_stats = bs.path_stats()
 
     # the same solve with the per-tuple interval culling in front of Newton
     bs.set_culling(True)
@@ -341,24 +346,46 @@ def bench_c2(args) -> dict:
         event_share[ev] = event_share.get(ev, 0) + n
     flops = _solver_flops(stats, event_share)
     dominant = max(kern, key=kern.get)
-    units = {}
+    units = {}  # kernel -> (paths, bytes per path, items, bytes per item)
     for K in (1, 2, 3):
         row = stats[f"K{K}"]
-        units[f"newton_k{K}"] = (row["items"], NEWTON_BYTES(K))
-        units[f"occlude_k{K}"] = (row["converged"], OCCLUDE_BYTES(K))
-        units[f"finish_k{K}"] = (row["converged"], FINISH_BYTES(K))
+        n_paths_k = int(ranges[K]) - int(ranges[K - 1])
+        units[f"newton_k{K}"] = (n_paths_k, *NEWTON_BYTES(K)[:1], row["items"], NEWTON_BYTES(K)[1])
+        units[f"occlude_k{K}"] = (path_stats[f"K{K}"], OCCLUDE_BYTES(K)[0], row["converged"], OCCLUDE_BYTES(K)[1])
+        units[f"finish_k{K}"] = (path_stats[f"K{K}"], FINISH_BYTES(K)[0], row["converged"], FINISH_BYTES(K)[1])
     kernel_name = {f"newton_k{K}": f"k_newton<{K}>" for K in (1, 2, 3)}
     kernel_name.update({f"occlude_k{K}": f"k_occlude<{K}>" for K in (1, 2, 3)})
     kernel_name.update({f"finish_k{K}": f"k_finish<{K}, false>" for K in (1, 2, 3)})
     kernel_name.update({"compact": "k_compact_scatter", "refine": "k_refine<K>", "cull": "k_cull_paths"})
     roof = {"bound": "hbm", "kernel": kernel_name[dominant], "kernel_ms": round(kern[dominant], 3)}
     if dominant in units:
-        n_units, per_unit = units[dominant]
-        achieved = n_units * per_unit / (kern[dominant] * 1e-3) / 1e9
+        n_p, per_p, n_i, per_i = units[dominant]
+        algo = n_p * per_p + n_i * per_i
+        achieved = algo / (kern[dominant] * 1e-3) / 1e9
         roof.update({"achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "bytes_per_unit": per_unit, "units": n_units})
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "algorithmic_bytes": algo,
+                     "bytes_per_path": per_p, "paths": n_p, "bytes_per_item": per_i, "items": n_i})
     traffic = profiled_traffic("c2", kernel_name[dominant])
     roof["traffic"] = traffic
+    # the occlusion kernels are traversal-bound: rays per second and the
+    # escape-link walk's work per ray (oracle counts on the sample)
+    ow = sample.get("occlusion_work")
+    if ow and ow["converged_items"]:
+        rays_per_item = ow["rays"] / ow["converged_items"]
+        traversal = {}
+        for K in (1, 2, 3):
+            t_ms = kern.get(f"occlude_k{K}", 0.0)
+            conv_k = stats[f"K{K}"]["converged"]
+            if t_ms > 0 and conv_k:
+                traversal[f"k_occlude<{K}>"] = {
+                    "converged_items_per_s": round(conv_k / (t_ms * 1e-3), 1),
+                    "rays_per_s_est": round(conv_k * rays_per_item / (t_ms * 1e-3), 1)}
+        traversal["sample"] = {"rays_per_converged_item": round(rays_per_item, 3),
+                               "boxes_per_ray": round(ow["boxes"] / max(ow["rays"], 1), 2),
+                               "triangles_per_ray": round(ow["triangles"] / max(ow["rays"], 1), 2),
+                               "walk": "escape-link binary walk (minimum_lbvh.h:1078-1152), oracle counts",
+                               "paths": ow["paths"]}
+        roof["traversal"] = traversal
     valu = {f"k_newton<{K}>": {"achieved_TFLOPs": round(flops[K] / (kern[f"newton_k{K}"] * 1e-3) / 1e12, 2),
                                "peak_TFLOPs": FP32_PEAK_TFLOPS,
                                "frac": round(flops[K] / (kern[f"newton_k{K}"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
@@ -421,6 +448,7 @@ def verify_sample(args, s, g, bs, d_res, n_all, L, st) -> dict:
     it (the CPU baseline; its FP32 results must equal the GPU's), and the FP64
     oracle measures the vertex error of the GPU's refined chains."""
     from palzilla_amd import caustics as CA
+    from palzilla_amd import scenes
     import oracle
 
     n_chunks = 64
@@ -458,6 +486,7 @@ def verify_sample(args, s, g, bs, d_res, n_all, L, st) -> dict:
     rng = np.random.default_rng(7)
     pick = conv if conv.size <= args.vertex_sample else rng.choice(conv, args.vertex_sample, replace=False)
     e32, e64 = [], []
+    vscale = scenes.vertex_error_scale(s)  # the specular geometry's diagonal
     for it in pick:
         pi, l = divmod(int(it), L)
         path = paths[pi]
@@ -472,10 +501,15 @@ def verify_sample(args, s, g, bs, d_res, n_all, L, st) -> dict:
         def verts(prm):
             p = np.asarray(prm, dtype=np.float64)[:2 * K].reshape(K, 2)
             return v[:, 0] + (v[:, 1] - v[:, 0]) * p[:, :1] + (v[:, 2] - v[:, 0]) * p[:, 1:]
-        sc = np.linalg.norm(x64, axis=1)
+        sc = vscale
         e32.append(np.max(np.linalg.norm(verts(want["params"][it]) - x64, axis=1) / sc))
         e64.append(np.max(np.linalg.norm(verts(res["params"][it]) - x64, axis=1) / sc))
     e32, e64 = np.asarray(e32), np.asarray(e64)
+    # occlusion-ray work of the solve (oracle counts, serial) on every 64th path of the sample
+    osub = paths[::64]
+    ow = oracle.occlusion_work(oracle.HostScene(s, nodes, root), osub)
+    ow["paths"] = int(osub.shape[0])
+    ow["converged_items"] = int((want["status"].reshape(-1, L)[::64] >= 1).sum())
     # photonTrace on the CPU (the oracle's stackfull traversal, 1 thread) over
     # every 64th triangle's photons, with its node and triangle counts
     stride = 64
@@ -483,6 +517,7 @@ def verify_sample(args, s, g, bs, d_res, n_all, L, st) -> dict:
     work = oracle.photon_trace_work(s, nodes, root, stride=stride)
     dt_trace = time.perf_counter() - t0
     return {
+        "occlusion_work": ow,
         "photon_trace_cpu": {"value": round(work["photons"] / dt_trace, 1), "unit": "photons/s", "cores": 1,
                              "kind": "port", "sample": f"oracle photonTrace of every {stride}th triangle's photons "
                                                        f"({work['photons']} photons, {dt_trace:.2f} s)",

@@ -592,6 +592,11 @@ int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, u
  * (K = 4).
  * Synchronises the stream. */
 int palz_batched_stats(palz_batched* b, uint64_t* stats16, void* stream);
+/* Per chain length K = 1..3: the number of paths of the last solve with at
+ * least one converged (path, wavelength) item, i.e. the paths whose records,
+ * triangles and receiver point the finishing kernels read.  No counterpart in
+ * the reference (its solver keeps no statistics).  Synchronises the stream. */
+int palz_batched_path_stats(palz_batched* b, uint64_t* converged_paths3, void* stream);
 /* Kernel timing of eager solves (graph launches are not timed): when
  * enabled, each palz_batched_solve records HIP events on its stream around
  * its kernels; palz_batched_timings waits for the last solve's events and

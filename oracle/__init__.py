@@ -47,6 +47,8 @@ _SIGS = {
     "orc_cmf_y_sample": (C.c_float, [C.c_float]),
     "orc_cmf_y_pdf": (C.c_float, [C.c_float]),
     "orc_solve_primary": (None, [_P, _P, _P, C.c_uint32, _P, _P, _P, _P, _P]),
+    "orc_occlusion_work": (None, [_P, _P, C.c_uint32, _P]),
+    "orc_fixed_word": (C.c_uint64, [C.c_float]),
     "orc_fresnel": (C.c_float, [_P, _P, C.c_float]),
     "orc_refraction": (C.c_int, [_P, _P, C.c_float, _P]),
     "orc_dual_optics": (None, [C.c_int, _P, _P, C.c_float, _P]),
@@ -269,6 +271,19 @@ def solve_paths(hs: HostScene, paths: np.ndarray, threads: int = 1, want_results
     res = np.zeros(paths.shape[0] * hs.scene.lambdas.shape[0], dtype=CA.PATH_RESULT) if want_results else None
     lib().orc_solve_paths(C.cast(C.pointer(hs.desc), C.c_void_p), _ptr(paths), paths.shape[0], _ptr(res), threads)
     return res
+
+
+def occlusion_work(hs: HostScene, paths: np.ndarray) -> dict:
+    """Work of contributableThroughput's occlusion rays (pk.h:88-115, :829-840)
+    over every (path, wavelength) item, solved serially: rays cast, child boxes
+    tested by the escape-link walk and triangles tested."""
+    paths = np.ascontiguousarray(paths)
+    out = np.zeros(16, dtype=np.uint64)
+    lib().orc_occlusion_work(C.cast(C.pointer(hs.desc), C.c_void_p), _ptr(paths), paths.shape[0], _ptr(out))
+    seg = [{"rays": int(out[4 + 3 * i]), "boxes": int(out[5 + 3 * i]), "triangles": int(out[6 + 3 * i])}
+           for i in range(4) if out[4 + 3 * i]]
+    return {"rays": int(out[0]), "boxes": int(out[1]), "triangles": int(out[2]), "items": int(out[3]),
+            "segments": seg}
 
 
 def solve_f64(scene, path, p_end, eta: float, params0, tol: float = 1e-28, max_iter: int = 64):

@@ -424,19 +424,32 @@ void intersect_stackfull(Hit* hit, const Scene& s, float3 ro, float3 rd, float3 
     }
 }
 
+// Work of the occlusion rays, counted when set (orc_occlusion_work): [0] rays,
+// [1] child boxes tested, [2] triangles tested, then the same three per
+// segment of the chain (light side first), 4 segments at most.
+thread_local uint64_t* g_occl = nullptr;
+thread_local int g_occl_segment = 0;
+inline void count_occl(int what) {
+    ++g_occl[what];
+    if (g_occl_segment < 4) ++g_occl[4 + 3 * g_occl_segment + what];
+}
+
 // intersect_escape_link, any-hit (a leaf root is tested directly)
 bool any_hit(const Scene& s, float3 ro, float3 rd, float3 ird, float t_max) {
     Hit hit;
     hit.t = t_max;
     float t, u, v;
     float3 ng;
+    if (g_occl) count_occl(0);
     if (s.root & kLeaf) {
+        if (g_occl) count_occl(2);
         const palz_triangle& tri = s.tris[s.root & ~kLeaf];
         return intersectRayTriangle(&t, &u, &v, &ng, 0.0f, hit.t, ro, rd, load3(tri.vs[0]), load3(tri.vs[1]),
                                     load3(tri.vs[2]));
     }
     uint32_t curr = s.root, prev = kInvalid;
     while (curr != kInvalid) {
+        if (g_occl) count_occl((curr & kLeaf) ? 2 : 1);
         if (curr & kLeaf) {
             const palz_triangle& tri = s.tris[curr & ~kLeaf];
             if (intersectRayTriangle(&t, &u, &v, &ng, 0.0f, hit.t, ro, rd, load3(tri.vs[0]), load3(tri.vs[1]),
@@ -565,8 +578,10 @@ float contributableThroughput(const float* parameters, float3 p_beg, float3 p_en
         float3 d = vertices[i] - vertices[i + 1];
         if (dot(d, d) < 1.0e-5f) return 0.0f;
     }
-    for (int i = 0; i < K + 1; i++)
+    for (int i = 0; i < K + 1; i++) {
+        g_occl_segment = i;
         if (occluded(s, vertices[i], shadingNormals[i], vertices[i + 1], shadingNormals[i + 1])) return 0.0f;
+    }
     return throughput;
 }
 
@@ -888,6 +903,15 @@ uint32_t orc_lookup_paths(const palz_photon_record* rec, uint32_t n_rec, float s
 // (ParallelFor, main.cpp:1745); adds contributions into sd->image (host) in
 // item order, or as 32.32 fixed point into sd->image_fixed when that is set,
 // and writes results if not NULL.
+// The 32.32 fixed-point word of one contribution: c * 2^32 truncated, 0 for a
+// NaN or non-positive c, saturated at the largest float below 2^64 for
+// c >= 2^32 (where the conversion would be undefined).
+uint64_t orc_fixed_word(float c) {
+    if (!(c > 0.0f)) return 0;
+    const float s = c < 4294967296.0f ? c * 4294967296.0f : 18446742974197923840.0f;
+    return (uint64_t)s;
+}
+
 void orc_solve_paths(const palz_scene_desc* sd, const palz_path* paths, uint32_t n_paths, palz_path_result* results,
                      int threads) {
     SolveCtx c = make_ctx(sd);
@@ -910,10 +934,25 @@ void orc_solve_paths(const palz_scene_desc* sd, const palz_path* paths, uint32_t
         if (res[it].status != PALZ_PATH_CONTRIBUTES) continue;
         const size_t idx = (size_t)(it % c.n_lambdas) * sd->n_points + paths[it / c.n_lambdas].pixel;
         if (sd->image_fixed)  // unsigned 32.32 fixed point, contribution truncated
-            sd->image_fixed[idx] += (uint64_t)(res[it].contribution * 4294967296.0f);
+            sd->image_fixed[idx] += orc_fixed_word(res[it].contribution);
         else
             sd->image[idx] += res[it].contribution;
     }
+}
+
+// The occlusion rays' work over every (path, wavelength) item of the paths,
+// solved serially: out[0..3] = {rays, child boxes tested, triangles tested,
+// items}, out[4 + 3 i .. 6 + 3 i] = {rays, boxes, triangles} of chain segment
+// i (i = 0 from the light, 4 segments at most).
+void orc_occlusion_work(const palz_scene_desc* sd, const palz_path* paths, uint32_t n_paths, uint64_t* out) {
+    SolveCtx c = make_ctx(sd);
+    for (int i = 0; i < 16; ++i) out[i] = 0;
+    g_occl = out;
+    palz_path_result r;
+    for (uint64_t it = 0; it < (uint64_t)n_paths * c.n_lambdas; it++)
+        solve_any(c, paths[it / c.n_lambdas], (uint32_t)(it % c.n_lambdas), &r);
+    g_occl = nullptr;
+    out[3] = (uint64_t)n_paths * c.n_lambdas;
 }
 
 // FP64 Newton of one chain from `params` (FP32 start), to cost < tol64, with

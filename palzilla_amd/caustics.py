@@ -362,6 +362,12 @@ class BatchedSolver:
         out["bad_scene"] = int(a[14])
         return out
 
+    def path_stats(self) -> dict:
+        """Per chain length: paths of the last solve with a converged item."""
+        s = (C.c_uint64 * 3)()
+        call("palz_batched_path_stats", C.c_void_p(self.handle), s, _sp(self.stream))
+        return {f"K{k + 1}": int(s[k]) for k in range(3)}
+
     def autotune(self, d_scenes: DeviceBuffer, n_scenes: int, n_lambdas: int, refine: int = 1) -> list[dict]:
         rep = (C.c_int32 * 18)()
         call("palz_batched_autotune", C.c_void_p(self.handle), C.c_void_p(d_scenes.ptr), C.c_uint32(n_scenes),

@@ -48,12 +48,14 @@ struct Counters {
     uint32_t pad_;
     unsigned long long stats[16];
     uint32_t occlude[3][kXcds];  // per chain length and XCD: list positions handed out
+    unsigned long long conv_paths[3];  // per chain length: paths with at least one converged item
 };
 
 __global__ void k_reset(Counters* c, const uint32_t* __restrict__ ranges, uint32_t L) {
     if (threadIdx.x < 3) c->ticket[threadIdx.x] = 0u;
     if (threadIdx.x == 0) c->n_items = ranges[4] * L;
     if (threadIdx.x < 16) c->stats[threadIdx.x] = 0ull;
+    if (threadIdx.x < 3) c->conv_paths[threadIdx.x] = 0ull;
 }
 
 // words[i] = chain length | i << 32
@@ -512,15 +514,30 @@ __global__ void __launch_bounds__(256) k_skipped(const uint32_t* __restrict__ ra
 }
 
 // Converged items, listed in item order from the exclusive scan of their flags.
+// Also counts, per chain length, the paths with at least one converged item
+// (an item is its path's first converged one when the scan has not moved
+// since the path's first item).
 __global__ void __launch_bounds__(256) k_compact_scatter(const uint32_t* __restrict__ excl,
-                                                         const Counters* __restrict__ ctr,
+                                                         Counters* __restrict__ ctr,
+                                                         const uint32_t* __restrict__ ranges, uint32_t L,
                                                          uint32_t* __restrict__ list) {
     const uint32_t n = ctr->n_items, total = ctr->converged;
+    unsigned long long first[3] = {0ull, 0ull, 0ull};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t p = excl[i];
         const uint32_t next = i + 1 < n ? excl[i + 1] : total;
-        if (next != p) list[p] = i;
+        if (next != p) {
+            list[p] = i;
+            const uint32_t pi = i / L;
+            if (excl[pi * L] == p) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    if (ranges[k] <= pi && pi < ranges[k + 1]) ++first[k];
+            }
+        }
     }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) wave_add_stat(&ctr->conv_paths[k], first[k]);
 }
 
 template <int K>
@@ -671,7 +688,7 @@ struct palz_batched {
         using namespace palz;
         k_skipped<<<256, 256, 0, st>>>(ranges, L, ctr, flags, results);
         launch_exclusive_scan(flags, flags, (uint32_t)cap_items, &ctr->n_items, &ctr->converged, scan_sums, st);
-        k_compact_scatter<<<cus * 8, 256, 0, st>>>(flags, ctr, list);
+        k_compact_scatter<<<cus * 8, 256, 0, st>>>(flags, ctr, ranges, L, list);
     }
 
     int launch_all(const palz_scene_desc* scenes, uint32_t n_scenes, uint32_t L, int refine,
@@ -880,6 +897,15 @@ int palz_batched_stats(palz_batched* b, uint64_t* stats16, void* stream) {
     palz::Counters h;
     PALZ_HIP(hipMemcpy(&h, b->ctr, sizeof(h), hipMemcpyDeviceToHost));
     for (int i = 0; i < 16; ++i) stats16[i] = h.stats[i];
+    return PALZ_OK;
+}
+
+int palz_batched_path_stats(palz_batched* b, uint64_t* converged_paths3, void* stream) {
+    if (!b || !converged_paths3) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_path_stats: NULL");
+    PALZ_HIP(hipStreamSynchronize((hipStream_t)stream));
+    palz::Counters h;
+    PALZ_HIP(hipMemcpy(&h, b->ctr, sizeof(h), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; ++k) converged_paths3[k] = h.conv_paths[k];
     return PALZ_OK;
 }
 

@@ -236,15 +236,26 @@ PALZ_HD void solve_qr(F* A, F* b, F* x) {
 }
 
 // ---------------------------------------------------------------------------
+// The 32.32 fixed-point word of a contribution: c * 2^32 truncated.  The
+// conversion is defined for 0 <= c * 2^32 < 2^64 only, so a NaN or
+// non-positive c gives 0 and c >= 2^32 saturates at the largest float below
+// 2^64 (with the radiance clamp checked at the API neither occurs).
+PALZ_HD inline unsigned long long fixed_word(float c) {
+    if (!(c > 0.0f)) return 0ull;
+    const float s = c < 4294967296.0f ? c * 4294967296.0f : 18446742974197923840.0f;
+    return (unsigned long long)s;
+}
+
 // One contribution into a scene's irradiance image: float atomics (the
 // reference's, main_gpu.cu:473-483) or, when the scene names a fixed-point
-// image, a 64-bit integer atomic of c * 2^32 truncated (order-independent).
+// image, a 64-bit integer atomic of its fixed-point word (order-independent).
 __device__ __forceinline__ void add_contribution(const palz_scene_desc& sc, size_t idx, float c) {
-    if (sc.image_fixed)
-        atomicAdd(reinterpret_cast<unsigned long long*>(sc.image_fixed) + idx,
-                  (unsigned long long)(c * 4294967296.0f));
-    else
+    if (sc.image_fixed) {
+        const unsigned long long w = fixed_word(c);
+        if (w) atomicAdd(reinterpret_cast<unsigned long long*>(sc.image_fixed) + idx, w);
+    } else {
         atomicAdd(sc.image + idx, c);
+    }
 }
 
 // EventDescriptor (pk.h:122-162): bits [0,24) events (1 = T), [24,32) size.

@@ -241,6 +241,8 @@ int palz_renderer_set_params(palz_renderer* r, const palz_renderer_params* p) {
     if (p->photons_per_triangle == 0 || p->path_capacity == 0 || p->refine < PALZ_REFINE_NONE ||
         p->refine > PALZ_REFINE_ALL)
         return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_renderer_set_params: bad photon budget, capacity or refine");
+    if (!(p->radiance_clamp > 0.0f && p->radiance_clamp < 2147483648.0f))
+        return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_renderer_set_params: radiance_clamp must be in (0, 2^31)");
     r->params = *p;
     return PALZ_OK;
 }

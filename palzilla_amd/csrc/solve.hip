@@ -108,6 +108,8 @@ int palz_solve_specular(int K, uint32_t event, const palz_scene_desc* scene, con
         return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_solve_specular: K must be 1..4 and match the event size");
     if (scene->n_lambdas == 0 || !scene->lambdas || !scene->image || !scene->points)
         return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_solve_specular: scene needs points, wavelengths and an image");
+    if (scene->image_fixed && !(scene->radiance_clamp > 0.0f && scene->radiance_clamp < 2147483648.0f))
+        return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_solve_specular: a fixed-point image needs radiance_clamp in (0, 2^31)");
     const uint64_t items = (uint64_t)n_paths * scene->n_lambdas;
     if (items == 0) return PALZ_OK;
     if (items >= (1ull << 31)) return fail(PALZ_ERR_CAPACITY, "palz_solve_specular: more than 2^31 items");

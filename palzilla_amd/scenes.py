@@ -186,6 +186,16 @@ def stack_scene(*, mirror_cells=0, layer="none", layer_cells=0, ior=1.5, cauchy_
                               "ior": ior, "photons": photons, "seed": seed})
 
 
+def vertex_error_scale(scene: CausticScene) -> float:
+    """The length relative vertex errors are measured against: the diagonal of
+    the bounding box of the scene's specular triangles (where chain vertices
+    lie).  It moves with the geometry, so the relative error does not depend
+    on where the scene sits in world space."""
+    spec = scene.attribs["material"] != T.MATERIAL_DIFFUSE
+    v = scene.tris[spec].reshape(-1, 3).astype(np.float64)
+    return float(np.linalg.norm(v.max(axis=0) - v.min(axis=0)))
+
+
 def with_occluders(scene: CausticScene, squares) -> CausticScene:
     """The scene with diffuse occluder squares appended after its triangles
     (so every existing triangle keeps its index and a path list made for the

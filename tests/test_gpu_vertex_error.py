@@ -2,8 +2,10 @@
 configuration: the FP32 Newton solution (the reference's stopping rule,
 cost < 1e-10) and the solution after the one FP64 refinement step, on up to
 256 converged chains of each of C1 (full size), the first 12 scenes of C4 (real
-sizes) and C5 (full size).  The error of a chain is the largest relative
-distance of its vertices from those of the FP64 oracle's converged chain."""
+sizes) and C5 (full size).  The error of a chain is the largest distance of
+its vertices from those of the FP64 oracle's converged chain, relative to the
+diagonal of the scene's specular geometry (scenes.vertex_error_scale), so it
+does not depend on where the scene sits in world space."""
 import numpy as np
 import pytest
 
@@ -67,7 +69,7 @@ def _vertex_errors(scene_list, paths, r32, r64, L, n_pick=256):
             p = np.asarray(prm, dtype=np.float64)[:2 * K].reshape(K, 2)
             return v[:, 0] + (v[:, 1] - v[:, 0]) * p[:, :1] + (v[:, 2] - v[:, 0]) * p[:, 1:]
 
-        sc = np.linalg.norm(x64, axis=1)
+        sc = scenes.vertex_error_scale(s)
         e32.append(np.max(np.linalg.norm(verts(r32["params"][it]) - x64, axis=1) / sc))
         e64.append(np.max(np.linalg.norm(verts(r64["params"][it]) - x64, axis=1) / sc))
         ks.append(K)

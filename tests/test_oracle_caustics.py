@@ -222,3 +222,22 @@ def test_fresnel_and_refraction_match_the_textbook_forms():
         woh = wo.astype(np.float64) / np.linalg.norm(wo)
         assert np.linalg.norm(woh - t) < 1e-5, (wi, n, eta)
     assert n_tir > 100
+
+
+def test_fixed_point_word_is_defined_at_the_edges():
+    """The 32.32 fixed-point image word (c * 2^32 truncated): exact for
+    ordinary contributions; 0 for NaN, negative and zero; saturated (not
+    wrapped or undefined) from 2^32 up, infinity included."""
+    w = oracle.lib().orc_fixed_word
+    f32 = np.float32
+    assert w(f32(1.5)) == 3 * 2 ** 31
+    assert w(f32(12.0)) == 12 * 2 ** 32
+    assert w(f32(2.0 ** -32)) == 1 and w(f32(2.0 ** -33)) == 0
+    for c in (float("nan"), -1.0, -0.0, 0.0, float("-inf")):
+        assert w(c) == 0, c
+    top = 18446742974197923840
+    assert top < 2 ** 64 and float(np.float32(top)) == top
+    for c in (2.0 ** 32, 2.0 ** 40, float("inf")):
+        assert w(c) == top, c
+    assert w(f32(np.nextafter(f32(2.0 ** 32), f32(0)))) == int(np.float64(np.nextafter(f32(2.0 ** 32), f32(0)))
+                                                               * 2 ** 32)
