@@ -24,6 +24,7 @@ _SIGS = {
     "orc_deltas": (None, [_P, C.c_int, _P]),
     "orc_build_lbvh": (None, [_P, C.c_int, _P, _P, _P, _P]),
     "orc_lbvh": (None, [_P, C.c_int, C.c_uint32, C.c_int, _P, _P, _P, _P, _P]),
+    "orc_lbvh_parallel": (None, [_P, C.c_int, C.c_uint32, C.c_int, _P, _P, C.c_int]),
     "orc_lbvh_topdown": (None, [_P, C.c_int, _P, _P, _P, _P]),
     "orc_validate_lbvh": (C.c_int, [_P, C.c_uint32, _P]),
     "orc_normal_bounds": (None, [_P, C.c_int, _P, _P]),
@@ -118,6 +119,18 @@ def lbvh(tris: np.ndarray, option: int = 0, bits: int = 32):
     lib().orc_lbvh(_ptr(tris), n, option, bits, _ptr(nodes), _ptr(root), _ptr(words), _ptr(deltas),
                    _ptr(box))
     return nodes[: n - 1], int(root[0]), words, deltas[: n - 1], box[0]
+
+
+def lbvh_parallel(tris: np.ndarray, option: int = 0, bits: int = 32, threads: int = 8):
+    """The LBVH pipeline on host threads (stable radix sort, parallel climb):
+    (nodes, root), equal to lbvh()'s."""
+    from palzilla_amd import types as T
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    n = tris.shape[0]
+    nodes = np.zeros(max(n - 1, 1), dtype=T.INTERNAL_NODE)
+    root = np.zeros(1, dtype=np.uint32)
+    lib().orc_lbvh_parallel(_ptr(tris), n, option, bits, _ptr(nodes), _ptr(root), threads)
+    return nodes[:max(n - 1, 0)], int(root[0])
 
 
 def lbvh_topdown(tris: np.ndarray, sorted_words: np.ndarray, deltas: np.ndarray):

@@ -1,7 +1,9 @@
 // Oracle: scene AABB, Morton keys, stable sort, deltas, Apetrei LBVH build,
 // an independent top-down LBVH build and validate_lbvh.  TEST INFRASTRUCTURE.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "oracle.hpp"
@@ -238,6 +240,102 @@ void orc_lbvh(const palz_triangle* tris, int n, uint32_t option, int bits, palz_
     if (words_out) std::memcpy(words_out, words.data(), sizeof(uint64_t) * n);
     if (deltas_out && n > 1) std::memcpy(deltas_out, d.data(), (size_t)n - 1);
     if (scene_out) *scene_out = scene;
+}
+
+// The same pipeline on `threads` host threads, the BVHCPUBuilder::build shape
+// with CPU_PARALLEL (minimum_lbvh.h:592-606): scene box, keys and deltas in
+// slices, a stable LSD radix sort of the (key, index) words with 8-bit digits
+// (per-thread digit counts, then a stable scatter of each slice), and the
+// bottom-up climb with one thread per leaf slice, where an atomic exchange on
+// the parent's arrival word (acquire-release) decides which child completes
+// it, as the reference's InterlockedExchange does (minimum_lbvh.h:340-344).
+// The result equals orc_lbvh's bit for bit: the tree is fixed by the sorted
+// keys, and boxes are exact min/max.
+void orc_lbvh_parallel(const palz_triangle* tris, int n, uint32_t option, int bits, palz_internal_node* nodes,
+                       uint32_t* root, int threads) {
+    threads = std::max(1, threads);
+    auto par = [&](auto&& body) {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t)
+            pool.emplace_back(body, (size_t)n * t / threads, (size_t)n * (t + 1) / threads, t);
+        for (auto& th : pool) th.join();
+    };
+    // scene box
+    std::vector<palz_aabb> part(threads);
+    par([&](size_t b, size_t e, int t) { orc_scene_aabb(tris + b, (int)(e - b), &part[t]); });
+    palz_aabb scene = part[0];
+    for (int t = 1; t < threads; ++t)
+        if (part[t].lower.x <= part[t].upper.x) scene = box_union(scene, part[t]);
+    // keys as (key, index) words
+    std::vector<uint64_t> words(n), tmp(n);
+    par([&](size_t b, size_t e, int) {
+        std::vector<uint32_t> k(e - b);
+        orc_morton_keys(tris + b, (int)(e - b), &scene, option, bits, k.data());
+        for (size_t i = b; i < e; ++i) words[i] = (uint64_t)k[i - b] | ((uint64_t)i << 32);
+    });
+    // stable LSD radix sort on bits [0, 32)
+    std::vector<uint32_t> counts((size_t)threads * 256);
+    for (int pass = 0; pass < 4; ++pass) {
+        const int shift = 8 * pass;
+        par([&](size_t b, size_t e, int t) {
+            uint32_t* c = &counts[(size_t)t * 256];
+            std::fill(c, c + 256, 0u);
+            for (size_t i = b; i < e; ++i) ++c[(words[i] >> shift) & 0xFFu];
+        });
+        // exclusive offsets, digit-major then thread order (stability)
+        uint32_t acc = 0;
+        for (int d = 0; d < 256; ++d)
+            for (int t = 0; t < threads; ++t) {
+                const uint32_t c = counts[(size_t)t * 256 + d];
+                counts[(size_t)t * 256 + d] = acc;
+                acc += c;
+            }
+        par([&](size_t b, size_t e, int t) {
+            uint32_t* off = &counts[(size_t)t * 256];
+            for (size_t i = b; i < e; ++i) tmp[off[(words[i] >> shift) & 0xFFu]++] = words[i];
+        });
+        words.swap(tmp);
+    }
+    std::vector<uint8_t> d(n > 1 ? n - 1 : 1);
+    par([&](size_t b, size_t e, int) {
+        for (size_t i = b; i < e && i + 1 < (size_t)n; ++i)
+            d[i] = (uint8_t)delta_of((uint32_t)words[i], (uint32_t)words[i + 1]);
+    });
+    // bottom-up climb
+    const uint32_t n_int = (uint32_t)n - 1u;
+    std::vector<std::atomic<uint32_t>> arrival(n_int > 0 ? n_int : 1);
+    for (auto& a : arrival) a.store(0xFFFFFFFFu, std::memory_order_relaxed);
+    std::atomic<uint32_t> root_node{0xFFFFFFFFu};
+    if (n == 1) root_node = (uint32_t)(words[0] >> 32) | kLeaf;
+    par([&](size_t b, size_t e, int) {
+        for (uint32_t leaf = (uint32_t)b; leaf < (uint32_t)e && n > 1; ++leaf) {
+            uint32_t lo = leaf, hi = leaf;
+            uint32_t node = (uint32_t)(words[leaf] >> 32) | kLeaf;
+            palz_aabb box = tri_box(tris[node & ~kLeaf]);
+            for (;;) {
+                const uint32_t dl = lo > 0 ? d[lo - 1] : 0xFFFFFFFFu;
+                const uint32_t dr = hi < n_int ? d[hi] : 0xFFFFFFFFu;
+                const int go_left = dl < dr ? 1 : 0;
+                const uint32_t parent = go_left ? lo - 1 : hi;
+                nodes[parent].children[go_left] = node;
+                nodes[parent].aabbs[go_left] = box;
+                if (!(node & kLeaf)) nodes[node].parent = parent;
+                const uint32_t other = arrival[parent].exchange(go_left ? hi : lo, std::memory_order_acq_rel);
+                if (other == 0xFFFFFFFFu) break;
+                lo = std::min(lo, other);
+                hi = std::max(hi, other);
+                nodes[parent].context = hi < n_int ? hi : 0xFFFFFFFFu;
+                node = parent;
+                box = box_union(box, nodes[parent].aabbs[go_left ^ 1]);
+                if (hi - lo == n_int) {
+                    nodes[node].parent = kInvalid;
+                    root_node.store(node);
+                    break;
+                }
+            }
+        }
+    });
+    *root = root_node.load();
 }
 
 void orc_lbvh_topdown(const palz_triangle* tris, int n, const uint64_t* sorted_words, const uint8_t* deltas,

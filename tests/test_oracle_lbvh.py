@@ -209,3 +209,16 @@ def test_normal_bounds_brute_force(n):
             box = nb["normal_bounds"][i][c]
             assert np.array_equal(box["lower"], sn.min(axis=0)), (i, c)
             assert np.array_equal(box["upper"], sn.max(axis=0)), (i, c)
+
+
+@pytest.mark.parametrize("n,threads", [(2, 4), (257, 3), (4096, 8), (100_003, 8), (300_000, 16)])
+def test_parallel_cpu_build_equals_the_serial_one(n, threads):
+    # the threaded CPU baseline (radix sort + atomic climb) builds the same tree
+    tris = scenes.triangle_soup(n, seed=n)
+    if n > 1000:
+        tris[n // 3: n // 3 + 500] = tris[n // 3]  # a long run of equal keys
+    for bits in (30, 32):
+        want, root, *_ = oracle.lbvh(tris, bits=bits)
+        got, groot = oracle.lbvh_parallel(tris, bits=bits, threads=threads)
+        assert groot == root
+        assert got.tobytes() == want.tobytes()
