@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# PALZ_ORACLE_LIB names another build of the same sources (the sanitizer test's)
+LIB_PATH = os.environ.get("PALZ_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 _P = C.c_void_p
 _SIGS = {

@@ -30,5 +30,23 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+SANITIZED = os.path.join(HERE, "_build", "liboracle_asan.so")
+
+
+def build_sanitized(verbose: bool = False) -> str:
+    """The same sources under AddressSanitizer and UndefinedBehaviorSanitizer
+    (host code, -O1, frame pointers kept), for the sanitizer test; load it in
+    a process that preloads libasan (see tests/test_sanitizers.py)."""
+    os.makedirs(os.path.dirname(SANITIZED), exist_ok=True)
+    flags = [f for f in FLAGS if f != "-O3"] + ["-O1", "-g", "-fno-omit-frame-pointer",
+                                                  "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined"]
+    cmd = ["g++", *flags, "-o", SANITIZED + ".tmp", *[os.path.join(HERE, s) for s in SOURCES]]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(SANITIZED + ".tmp", SANITIZED)
+    return SANITIZED
+
+
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build_sanitized(verbose=True) if "--sanitize" in sys.argv else build(verbose="-v" in sys.argv))
