@@ -146,6 +146,18 @@ int palz_sort_u64(palz_sorter* s, uint64_t* words, uint64_t* tmp, uint32_t* valu
 int palz_sort_u32(palz_sorter* s, uint32_t* keys, uint32_t* tmp, uint32_t* values, uint32_t* tmp_values,
                   uint32_t n, int start_bit, int end_bit, void* stream);
 
+/* palz_sort_u64 without a sorter and without allocating: no counterpart in
+ * the reference.  For callers that cannot allocate (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as palz_sort_u64; the scratch (look-back
+ * table, digit histograms, tile tickets) is a file-scope device array of
+ * 8.4 MB that each device holds until the process ends.  n must be <= 2^24
+ * (PALZ_ERR_CAPACITY otherwise).  The first call on a device sets the array
+ * up and synchronises `stream`.  Every call on a device shares its array:
+ * calls that overlap on one device (other streams or threads) race. */
+int palz_sort_u64_noalloc(uint64_t* words, uint64_t* tmp, uint32_t* values, uint32_t* tmp_values, uint32_t n,
+                          int start_bit, int end_bit, void* stream);
+
 /* ---- LBVH builder ---------------------------------------------------------
  * Replaces minimum_lbvh::BVHGPUBuilder (minimum_lbvh.h:741-900).  The builder
  * owns the node array (n-1 records) and the root index, as the reference does

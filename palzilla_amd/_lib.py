@@ -49,6 +49,8 @@ _SIGS = {
                                 C.c_uint32, C.c_int, C.c_int, C.c_void_p]),
     "palz_sort_u32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_uint32, C.c_int, C.c_int, C.c_void_p]),
+    "palz_sort_u64_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int,
+                                        C.c_int, C.c_void_p]),
     "palz_lbvh_create": (C.c_int, [c_void_pp]),
     "palz_lbvh_destroy": (C.c_int, [C.c_void_p]),
     "palz_lbvh_build": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_int,
