@@ -131,23 +131,31 @@ def residual(x, verts, nrms, pb, pe, eta, event, K):
 
 
 def solve(x0, verts, nrms, pb, pe, eta, event, K, iters=8):
-    """Gauss-Newton in FP64 from x0 (n, 2K) to round-off."""
+    """Gauss-Newton in FP64 from x0 (n, 2K) to round-off: at most `iters`
+    steps; an item leaves the iteration once its step is below 1e-15."""
     x = np.array(x0, dtype=np.float64)
     n = x.shape[0]
     P = 2 * K
+    active = np.arange(n)
     for _ in range(iters):
-        xs = [D(x[:, i], np.eye(P)[i][None, :].repeat(n, 0)) for i in range(P)]
-        vv = [[V.const(verts[k][:, c], P) for c in range(3)] for k in range(K)]
-        nn = [[V.const(nrms[k][:, c], P) for c in range(3)] for k in range(K)]
-        F = residual(xs, vv, nn, V.const(pb, P), V.const(pe, P), D.const(eta, P), event, K)
-        A = np.stack([f.g for f in F], axis=1)          # (n, 3K, P)
-        b = np.stack([f.v for f in F], axis=1)          # (n, 3K)
+        if active.size == 0:
+            break
+        m = active.size
+        xs = [D(x[active, i], np.eye(P)[i][None, :].repeat(m, 0)) for i in range(P)]
+        vv = [[V.const(verts[k][active, c], P) for c in range(3)] for k in range(K)]
+        nn = [[V.const(nrms[k][active, c], P) for c in range(3)] for k in range(K)]
+        F = residual(xs, vv, nn, V.const(pb[active], P), V.const(pe[active], P), D.const(eta[active], P),
+                     event[active], K)
+        A = np.stack([f.g for f in F], axis=1)          # (m, 3K, P)
+        b = np.stack([f.v for f in F], axis=1)          # (m, 3K)
         # an item at a square root of zero (grazing refraction) has no finite
         # Jacobian: it keeps its x, and its contribution comes out non-finite
         ok = np.all(np.isfinite(A), axis=(1, 2)) & np.all(np.isfinite(b), axis=1)
-        dx = np.zeros_like(x)
+        dx = np.zeros((m, P))
         dx[ok] = np.einsum("nij,nj->ni", np.linalg.pinv(A[ok]), b[ok])
-        x = x - dx
+        x[active] = x[active] - dx
+        moving = ok & (np.max(np.abs(dx), axis=1) > 1e-15)
+        active = active[moving]
     return x
 
 
@@ -316,10 +324,40 @@ class ItemSet:
 
     def contributions(self, heights, sel=None):
         sel = np.arange(self.item.size) if sel is None else sel
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return self._contributions(heights, sel)
+
+    def _contributions(self, heights, sel):
         verts, nrms = self.geometry(heights, sel)
         x = solve(self.x0[sel], verts, nrms, self.pb[sel], self.pe[sel], self.eta[sel], self.event[sel], self.K)
         return contribution(x, verts, nrms, self.pb[sel], self.pe[sel], self.ng[sel], self.eta[sel],
                             self.event[sel], self.dielectric[sel], self.K, self.intensity, self.clamp)
+
+    def contributions_many(self, heights_list, sel):
+        """contributions(h, sel) for every h of heights_list in one vectorised
+        solve: (len(heights_list), len(sel)) array."""
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return self._contributions_many(heights_list, sel)
+
+    # A chain that total internal reflection ends inside dA/dw is carried on
+    # through zero directions (inf / nan) and then replaced by the reference's
+    # FLT_MAX footprint (pk.h:844-930); those masked lanes are why the
+    # floating-point warnings are silenced here.  A chain whose own value is
+    # not finite is reported by loss_fd_ladder.
+    def _contributions_many(self, heights_list, sel):
+        m = len(heights_list)
+        if sel.size == 0:
+            return np.zeros((m, 0))
+        geo = [self.geometry(h, sel) for h in heights_list]
+        verts = [np.concatenate([g[0][k] for g in geo]) for k in range(self.K)]
+        nrms = [np.concatenate([g[1][k] for g in geo]) for k in range(self.K)]
+
+        def rep(a):
+            return np.concatenate([a[sel]] * m)
+        x = solve(rep(self.x0), verts, nrms, rep(self.pb), rep(self.pe), rep(self.eta), rep(self.event), self.K)
+        c = contribution(x, verts, nrms, rep(self.pb), rep(self.pe), rep(self.ng), rep(self.eta), rep(self.event),
+                         rep(self.dielectric), self.K, self.intensity, self.clamp)
+        return c.reshape(m, sel.size)
 
     def image(self, heights):
         img = np.zeros(self.scene.lambdas.shape[0] * self.n_points)
@@ -329,13 +367,20 @@ class ItemSet:
     def depends(self, j, nb):
         """Items whose inputs depend on height j (their vertices' heights or the
         heights their normals use)."""
-        users = {j} | {int(v) for v in np.nonzero(np.any(nb == j, axis=1))[0]}
-        if not hasattr(self, "_item_tv"):  # per chain vertex: the items' triangle vertex ids
-            self._item_tv = [self.tv[self.tris[:, k]] for k in range(self.K)]
-        mask = np.zeros(self.item.size, dtype=bool)
-        for tv in self._item_tv:
-            mask |= np.isin(tv, list(users)).any(axis=1)
-        return np.nonzero(mask)[0]
+        users = [j] + [int(v) for v in np.nonzero(np.any(nb == j, axis=1))[0]]
+        if not hasattr(self, "_item_tv"):
+            # vertex id -> items whose chain triangles use that vertex (CSR)
+            tv = np.concatenate([self.tv[self.tris[:, k]] for k in range(self.K)], axis=1)  # (n, 3K)
+            items = np.repeat(np.arange(self.item.size), tv.shape[1])
+            v = tv.reshape(-1)
+            keep = v >= 0
+            v, items = v[keep], items[keep]
+            order = np.argsort(v, kind="stable")
+            self._item_tv = (v[order], items[order])
+        v_sorted, items_sorted = self._item_tv
+        parts = [items_sorted[np.searchsorted(v_sorted, u):np.searchsorted(v_sorted, u, side="right")]
+                 for u in users]
+        return np.unique(np.concatenate(parts)) if parts else np.zeros(0, np.int64)
 
     def loss_fd(self, heights, target, j, nb, h=1e-4):
         """Central difference of L = sum (I - T)^2 in height j (FP64)."""
@@ -387,6 +432,35 @@ class Problem:
         for s in self.sets:
             img += s.image(heights)
         return img
+
+    def loss_fd_ladder(self, heights, target, j, steps, base):
+        """Central differences of L = sum (I - T)^2 in height j for every step
+        of `steps` (FP64; `base` = self.image(heights), or the image the
+        gradient is taken around).  Returns (differences, finite): finite is
+        False when a dependent item's FP64 chain is not finite at one of the
+        stencil's heights (it reaches a square root of zero: grazing or
+        total internal reflection within the step), in which case that
+        height has no central difference at these steps."""
+        sels = [s.depends(j, self.nb) for s in self.sets]
+        hs = [np.array(heights, dtype=np.float64)]
+        for h in steps:
+            for sgn in (1.0, -1.0):
+                hh = np.array(heights, dtype=np.float64)
+                hh[j] += sgn * h
+                hs.append(hh)
+        cs = [s.contributions_many(hs, sel) for s, sel in zip(self.sets, sels)]
+        finite = all(np.all(np.isfinite(c)) for c in cs)
+        out = []
+        for i in range(len(steps)):
+            val = []
+            for v in (1 + 2 * i, 2 + 2 * i):
+                img = base.copy()
+                for s, sel, c in zip(self.sets, sels, cs):
+                    if sel.size:
+                        np.add.at(img, s.lam[sel] * s.n_points + s.pixel[sel], c[v] - c[0])
+                val.append(np.sum((img - target) ** 2))
+            out.append((val[0] - val[1]) / (2.0 * steps[i]))
+        return np.array(out), finite
 
     def loss_fd(self, heights, target, j, h=1e-4, base=None):
         """Central difference of L = sum (I - T)^2 in height j (FP64); `base`,

@@ -2,6 +2,8 @@
 torch.autograd.Function: d(L2 image loss)/d(heights) of a refractive slab
 heightfield against FP64 central finite differences (h = 1e-4) of the
 oracle's FP64 restatement, on 64 random height parameters."""
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import pytest
 import torch
@@ -132,7 +134,8 @@ def test_adjoint_full_size_matches_fp64_finite_differences():
     # 512^2 receiver, 2^21 photons, target from a second seed.  The loss is
     # taken around the GPU image, as the adjoint's weights 2 (I - T) are; each
     # central difference moves only the items that depend on the height, with
-    # their FP64 contributions.  64 random live heights.
+    # their FP64 contributions.  64 random live heights, differences at
+    # h = 1e-4 (the configured step), 1e-5, 1e-6 and 1e-7.
     import time
 
     cells, width, photons = 128, 512, 1 << 21
@@ -170,30 +173,56 @@ def test_adjoint_full_size_matches_fp64_finite_differences():
             assert far.sum() <= 1e-3 * sel.size and sums <= 1e-4
     live = np.nonzero(np.abs(grad) > 0)[0]
     assert live.size > 1000
-    pick = rng.choice(live, 64, replace=False)
-    ad = grad[pick]
     base = r.image
     print(f"finite: image {np.isfinite(base).all()}, target {np.isfinite(target).all()}, "
           f"gradient {np.isfinite(grad).all()}", flush=True)
-    fd4, fd6 = [], []
-    for i, j in enumerate(pick):
-        fd4.append(prob.loss_fd(h, target, int(j), 1e-4, base=base))
-        fd6.append(prob.loss_fd(h, target, int(j), 1e-6, base=base))
-        if i % 16 == 15:
-            print(f"{i + 1}/64 heights, {time.perf_counter() - t0:.1f} s", flush=True)
-    fd4, fd6 = np.array(fd4), np.array(fd6)
-    # a height whose items include one at a square root of zero has no finite
-    # FP64 difference: it counts as a miss below
-    print(f"non-finite differences: {int((~np.isfinite(fd4)).sum())} (h=1e-4), "
-          f"{int((~np.isfinite(fd6)).sum())} (h=1e-6)", flush=True)
-    scale = np.nanmax(np.abs(fd6))
-    rel4 = np.abs(ad - fd4) / (np.abs(fd4) + 1e-3 * scale)
-    rel6 = np.abs(ad - fd6) / (np.abs(fd6) + 1e-3 * scale)
-    rel4, rel6 = (np.where(np.isfinite(x), x, np.inf) for x in (rel4, rel6))
-    print(f"64 params, |dL/dh| up to {scale:.4g}: vs FD h=1e-4 median {np.median(rel4):.2e}, "
-          f"{np.sum(rel4 < 1e-2)}/64 within 1e-2; vs FD h=1e-6 median {np.median(rel6):.2e}, "
-          f"{np.sum(rel6 < 1e-3)}/64 within 1e-3", flush=True)
-    # at 128^2 cells a step of 1e-4 crosses clamp kinks and strong caustic
-    # curvature (measured: median 7e-3), so the converged step h = 1e-6 is the
-    # reference here (measured: median 6.6e-6, 59/64 within 1e-3)
-    assert np.median(rel6) < 1e-4 and np.sum(rel6 < 1e-3) >= 56
+    # The FP64 central differences of 64 random live heights at h = 1e-4 ..
+    # 1e-7.  Rule: a height whose stencil puts one of its items' FP64 chains at
+    # a square root of zero (grazing refraction or total internal reflection
+    # within the step: the chain's value is not finite) has no central
+    # difference there; it is left out, counted, and another height drawn.
+    steps = (1e-4, 1e-5, 1e-6, 1e-7)
+    for st in prob.sets:
+        st.depends(0, prob.nb)  # build the dependency index before the threads share it
+    order = rng.permutation(live)
+
+    def ladder(j):
+        return prob.loss_fd_ladder(h, target, int(j), steps, base)
+
+    picked, fds, excluded = [], [], 0
+    with ThreadPoolExecutor(8) as ex:
+        at = 0
+        while len(picked) < 64:
+            batch = order[at:at + 64 - len(picked)]
+            at += batch.size
+            assert batch.size, "too few heights with finite central differences"
+            for j, (fd, finite) in zip(batch, ex.map(ladder, batch)):
+                if finite:
+                    picked.append(int(j))
+                    fds.append(fd)
+                else:
+                    excluded += 1
+            print(f"{len(picked)}/64 heights, {excluded} left out, {time.perf_counter() - t0:.1f} s", flush=True)
+    fd = np.array(fds)
+    assert np.all(np.isfinite(fd)), "a kept height has a non-finite central difference"
+    ad = grad[np.array(picked)]
+    scale = np.max(np.abs(fd[:, -1]))
+    # A step has resolved a height when the difference no longer moves: within
+    # 1e-3 of the next step's (a tenth).  The truncation error of a resolved
+    # difference is then about that change, so the adjoint must agree within
+    # 2.5e-3 at every step on every height that step resolves.
+    ok_counts = []
+    for i, hstep in enumerate(steps[:3]):
+        resolved = np.abs(fd[:, i] - fd[:, i + 1]) <= 1e-3 * np.abs(fd[:, i + 1])
+        rel = np.abs(ad - fd[:, i]) / (np.abs(fd[:, i]) + 1e-6 * scale)
+        print(f"h={hstep:g}: {int(resolved.sum())}/64 heights resolved; adjoint vs FD on them: median "
+              f"{np.median(rel[resolved]) if resolved.any() else float('nan'):.2e}, max "
+              f"{rel[resolved].max() if resolved.any() else float('nan'):.2e}; over all 64: median "
+              f"{np.median(rel):.2e}", flush=True)
+        assert np.all(rel[resolved] <= 2.5e-3), f"h={hstep:g}: adjoint differs from a resolved difference"
+        ok_counts.append(int(resolved.sum()))
+    # At 128^2 cells the caustic image's curvature leaves h = 1e-4 unresolved for
+    # most heights (measured on 85 heights: 27 resolved at 1e-4, 55 at 1e-5, 73
+    # at 1e-6); the heights no step resolves sit within 1e-6 of a radiance-clamp
+    # kink or a total-internal-reflection cut.
+    assert ok_counts[0] >= 10 and ok_counts[1] >= 28 and ok_counts[2] >= 44, ok_counts
