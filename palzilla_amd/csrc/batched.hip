@@ -296,7 +296,7 @@ __device__ __forceinline__ void refine_step(const float* prm, v3 pb, v3 pe, cons
 // contributableThroughput (pk.h:760-842) of the converged chains of length K,
 // in item order: every test but the occlusion rays per lane, then the K + 1
 // occlusion rays as wave packets (per lane when the wave's items belong to
-// more than one scene).  Writes the throughput (0 when any test fails) per
+// more than one scene or to more than kPacketPaths paths).  Writes the throughput (0 when any test fails) per
 // list position.  A kernel of its own so that the latency-bound traversal runs
 // at the occupancy of its few live registers, not at that of dA/dw's.
 //
@@ -309,6 +309,8 @@ __device__ __forceinline__ void refine_step(const float* prm, v3 pb, v3 pe, cons
 // passes): large enough to keep the counters cold on big lists, small enough
 // that a short list still spreads over every wave.
 constexpr uint32_t kOccludeGrabMax = 256;
+// Most distinct paths in a wave for which the occlusion rays walk as one packet.
+constexpr uint32_t kPacketPaths = 16;
 
 // Clears chain length K's per-XCD counters before its k_occlude (a kernel, not
 // a memset, so that it can be captured in a graph).
@@ -365,12 +367,20 @@ __global__ void __launch_bounds__(256, 8)  // 8 waves per SIMD: latency-bound, f
                 }
                 const uint32_t scene0 = __builtin_amdgcn_readfirstlane(path.scene);
                 const bool one_scene = __ballot(path.scene != scene0) == 0ull;
+                // A packet walk visits the union of its lanes' nodes: cheap when
+                // the wave's items are the wavelengths of a few paths (their rays
+                // nearly coincide), costly when the lanes hold many different
+                // paths (one wavelength per path).  Those waves walk per lane.
+                const uint32_t prev_pi = (uint32_t)__shfl_up((int)pi, 1, 64);
+                const uint32_t n_paths_wave =
+                    (uint32_t)__popcll(__ballot(in_range && (lane == 0u || prev_pi != pi)));
+                const bool packet = one_scene && n_paths_wave <= kPacketPaths;
                 bool visible = 0.0f < thr;
 #pragma unroll
                 for (int i = 0; i < K + 1; ++i) {
                     const v3 ird = inv_rd(seg[i].rd);
                     bool hit;
-                    if (one_scene) {
+                    if (packet) {
                         const palz_scene_desc* us = scenes + scene0;
                         hit = us->wide ? any_hit_wide(us->nodes, us->wide, us->tris, *us->root, visible, seg[i].ro,
                                                       seg[i].rd, ird, 1.0f, pstack)
