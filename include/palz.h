@@ -237,6 +237,31 @@ typedef struct {
     uint32_t pad_;
 } palz_scene_view;
 
+/* minimum_lbvh::Hit (minimum_lbvh.h:905-911): t (the query's t_max when
+ * nothing is hit), barycentrics (u, v) of vertices 1 and 2, the unnormalised
+ * geometric normal cross(v1 - v0, v2 - v1), triangle index (0xFFFFFFFF when
+ * nothing is hit). */
+typedef struct {
+    float t;
+    float u, v;
+    palz_float3 ng;
+    uint32_t tri;
+} palz_hit;
+
+enum {
+    PALZ_QUERY_CLOSEST = 0,           /* intersect_stackfull (minimum_lbvh.h:927-988) */
+    PALZ_QUERY_CLOSEST_STACKFREE = 1, /* intersect_stackfree, RAY_QUERY_CLOSEST (:996-1076) */
+    PALZ_QUERY_ANY = 2                /* intersect_stackfree, RAY_QUERY_ANY: first hit found */
+};
+
+/* Casts n rays (d_ro, d_rd: 3 floats per ray, device memory) into the scene's
+ * BVH over [0, t_max] and writes one palz_hit per ray.  Replaces the
+ * reference's traversal device functions (minimum_lbvh.h:927-1076) for a
+ * caller that holds rays rather than a kernel of its own.  Closest-hit
+ * queries take t_max = FLT_MAX as the reference does. */
+int palz_intersect_rays(const palz_scene_view* view, const float* d_ro, const float* d_rd, uint32_t n, int query,
+                        float t_max, palz_hit* d_hits, void* stream);
+
 /* photonTrace arguments (main_gpu.cu:56, pk.h:1441-1459). */
 typedef struct {
     palz_float3 p_light;
@@ -597,6 +622,13 @@ int palz_batched_solve(palz_batched* b, const palz_scene_desc* d_scenes, uint32_
 int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, uint32_t n_scenes,
                              uint32_t n_lambdas, int refine, palz_path_result* d_results, double* d_refined,
                              void* stream);
+/* How a captured solve is laid out (no counterpart in the reference): 1 =
+ * three graph branches, one per chain length (forked after the reset, joined
+ * for the compaction, forked again for the finishing kernels), so that one
+ * length's short kernels overlap the others'; 0 = one stream; -1 (default) =
+ * branches when no chain length holds three quarters of the paths (read once,
+ * at capture).  Drops the captured graph. */
+int palz_batched_set_branches(palz_batched* b, int mode);
 /* Per chain length K = 1..3 (row K-1): [0] items, [1] converged, [2]
  * contributing, [3] residual evaluations (Newton iterations plus the final
  * converged check); row 3: [0] items culled, [2] items whose path names a

@@ -57,6 +57,7 @@ _SIGS = {
     "orc_iv_ops": (None, [C.c_int, _P, _P, _P]),
     "orc_ray_triangle": (C.c_int, [_P, _P, _P, C.c_float, C.c_float, _P]),
     "orc_closest_hits": (None, [_P, _P, C.c_uint32, _P, _P, C.c_uint32, _P, _P]),
+    "orc_intersect_rays": (None, [_P, _P, C.c_uint32, _P, _P, C.c_uint32, C.c_int, C.c_float, _P]),
     "orc_brute_hits": (None, [_P, C.c_uint32, _P, _P, C.c_uint32, C.c_float, _P, _P, _P]),
     "orc_any_hits": (None, [_P, _P, C.c_uint32, _P, _P, C.c_uint32, C.c_float, _P]),
     "orc_occluded": (None, [_P, _P, C.c_uint32, _P, _P, _P, _P, C.c_uint32, _P]),
@@ -445,6 +446,21 @@ def closest_hits(tris, nodes, root: int, ro, rd):
     tri = np.zeros(n, np.uint32)
     lib().orc_closest_hits(_ptr(tris), _ptr(nodes), root, _ptr(ro), _ptr(rd), n, _ptr(t), _ptr(tri))
     return t, tri
+
+
+HIT = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("ng", "<f4", 3), ("tri", "<u4")])
+
+
+def intersect_rays(tris, nodes, root: int, ro, rd, query: int = 0, t_max: float = 3.402823466e38) -> np.ndarray:
+    """palz_hit records of the oracle's traversals: query 0 stackfull closest,
+    1 stackfree closest, 2 stackfree any-hit (minimum_lbvh.h:927-1076)."""
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    ro = np.ascontiguousarray(ro, np.float32).reshape(-1, 3)
+    rd = np.ascontiguousarray(rd, np.float32).reshape(-1, 3)
+    out = np.zeros(ro.shape[0], HIT)
+    lib().orc_intersect_rays(_ptr(tris), _ptr(nodes), root, _ptr(ro), _ptr(rd), ro.shape[0], query, t_max,
+                             _ptr(out))
+    return out
 
 
 def brute_hits(tris, ro, rd, t_max: float = 1.0):

@@ -424,6 +424,53 @@ void intersect_stackfull(Hit* hit, const Scene& s, float3 ro, float3 rd, float3 
     }
 }
 
+// intersect_stackfree (minimum_lbvh.h:996-1076); `any` is RAY_QUERY_ANY.  A
+// leaf root is tested directly.
+void intersect_stackfree(Hit* hit, const Scene& s, float3 ro, float3 rd, float3 ird, bool any) {
+    uint32_t curr = s.root, prev = kInvalid;
+    while (curr != kInvalid) {
+        if (curr & kLeaf) {
+            const palz_triangle& tri = s.tris[curr & ~kLeaf];
+            float t, u, v;
+            float3 ng;
+            if (intersectRayTriangle(&t, &u, &v, &ng, 0.0f, hit->t, ro, rd, load3(tri.vs[0]), load3(tri.vs[1]),
+                                     load3(tri.vs[2]))) {
+                hit->t = t;
+                hit->u = u;
+                hit->v = v;
+                hit->ng = ng;
+                hit->tri = curr & ~kLeaf;
+                if (any) break;
+            }
+            std::swap(curr, prev);
+            continue;
+        }
+        const palz_internal_node& nd = s.nodes[curr];
+        float l0, l1, r0, r1;
+        slabs(ro, ird, load3(nd.aabbs[0].lower), load3(nd.aabbs[0].upper), hit->t, &l0, &l1);
+        slabs(ro, ird, load3(nd.aabbs[1].lower), load3(nd.aabbs[1].upper), hit->t, &r0, &r1);
+        const bool hitL = l0 <= l1, hitR = r0 <= r1;
+        uint32_t near_node = nd.children[0], far_node = nd.children[1];
+        int nHits = 0;
+        if (hitL && hitR) {
+            if (r0 < l0) std::swap(near_node, far_node);
+            nHits = 2;
+        } else if (hitL || hitR) {
+            nHits = 1;
+            near_node = hitR ? far_node : near_node;
+        }
+        uint32_t next;
+        if (prev == nd.parent)
+            next = 0 < nHits ? near_node : nd.parent;
+        else if (prev == near_node)
+            next = nHits == 2 ? far_node : nd.parent;
+        else
+            next = nd.parent;
+        prev = curr;
+        curr = next;
+    }
+}
+
 // Work of the occlusion rays, counted when set (orc_occlusion_work): [0] rays,
 // [1] child boxes tested, [2] triangles tested, then the same three per
 // segment of the chain (light side first), 4 segments at most.
@@ -1342,6 +1389,27 @@ void orc_closest_hits(const palz_triangle* tris, const palz_internal_node* nodes
         intersect_stackfull(&hit, s, o, d, invRd(d), nullptr);
         hit_t[i] = hit.t;
         hit_tri[i] = hit.tri;
+    }
+}
+// The ray queries of palz_intersect_rays: query 0 intersect_stackfull,
+// 1 intersect_stackfree (closest), 2 intersect_stackfree (any), over
+// [0, t_max]; one palz_hit per ray.
+void orc_intersect_rays(const palz_triangle* tris, const palz_internal_node* nodes, uint32_t root, const float* ro,
+                        const float* rd, uint32_t n, int query, float t_max, palz_hit* out) {
+    const Scene s{tris, nullptr, nodes, root};
+    for (uint32_t i = 0; i < n; ++i) {
+        const float3 o = {ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]}, d = {rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]};
+        Hit hit;
+        hit.t = t_max;
+        if (query == 0)
+            intersect_stackfull(&hit, s, o, d, invRd(d), nullptr);
+        else
+            intersect_stackfree(&hit, s, o, d, invRd(d), query == 2);
+        out[i].t = hit.t;
+        out[i].u = hit.u;
+        out[i].v = hit.v;
+        out[i].ng = {hit.ng.x, hit.ng.y, hit.ng.z};
+        out[i].tri = hit.tri;
     }
 }
 // Brute force over every triangle, no BVH: the nearest hit (hit_t, hit_tri as

@@ -109,6 +109,9 @@ _SIGS = {
                                            C.c_void_p, C.c_void_p]),
     "palz_batched_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p]),
     "palz_batched_path_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p]),
+    "palz_batched_set_branches": (C.c_int, [C.c_void_p, C.c_int]),
+    "palz_intersect_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_float,
+                                      C.c_void_p, C.c_void_p]),
     "palz_batched_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "palz_batched_timings": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "palz_batched_set_culling": (C.c_int, [C.c_void_p, C.c_int]),

@@ -21,7 +21,7 @@ LIB = os.path.join(PKG, "libpalz.so")
 SOURCES = [  # the longest compiles first: they run in parallel
     "pathcuts_k3.hip", "pathcuts_k3_records.hip", "pathcuts_k2.hip", "adjoint.hip", "batched.hip", "lbvh.hip",
     "pathcuts_k1.hip", "primary.hip", "solve.hip", "cull.hip", "trace.hip", "sort.hip", "prims.hip", "spectral.hip",
-    "runtime.hip", "pathcuts.hip", "scene_io.hip", "renderer.hip", "sort_noalloc.hip"]
+    "runtime.hip", "pathcuts.hip", "scene_io.hip", "renderer.hip", "sort_noalloc.hip", "rays.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

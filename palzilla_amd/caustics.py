@@ -236,6 +236,22 @@ class GpuScene:
              C.c_uint32(n_paths), C.c_void_p(None), C.c_void_p(d_results.ptr if d_results else None),
              _sp(self.stream))
 
+    def intersect_rays(self, ro: np.ndarray, rd: np.ndarray, query: int = 0,
+                       t_max: float = 3.402823466e38) -> np.ndarray:
+        """palz_intersect_rays over host rays (n x 3 origins and directions):
+        query 0 stackfull closest hit, 1 stackfree closest hit, 2 any hit;
+        returns the palz_hit records."""
+        from .types import HIT
+        ro = np.ascontiguousarray(ro, np.float32).reshape(-1, 3)
+        rd = np.ascontiguousarray(rd, np.float32).reshape(-1, 3)
+        n = ro.shape[0]
+        d_ro, d_rd = DeviceBuffer.from_array(ro), DeviceBuffer.from_array(rd)
+        d_hits = DeviceBuffer(max(n, 1) * HIT.itemsize)
+        v = self.view()
+        call("palz_intersect_rays", C.byref(v), C.c_void_p(d_ro.ptr), C.c_void_p(d_rd.ptr), C.c_uint32(n),
+             int(query), C.c_float(t_max), C.c_void_p(d_hits.ptr), _sp(self.stream))
+        return d_hits.download(HIT, n, self.stream)
+
     def image(self) -> np.ndarray:
         return self.d_image.download(np.float32, self.n_points * self.n_lambdas, self.stream).reshape(
             self.n_lambdas, self.scene.height, self.scene.width)
@@ -361,6 +377,11 @@ class BatchedSolver:
         out["culled"] = int(a[12])
         out["bad_scene"] = int(a[14])
         return out
+
+    def set_branches(self, mode: int) -> None:
+        """Captured solves as three graph branches (1), one stream (0) or
+        decided from the chain-length mix (-1, the default)."""
+        call("palz_batched_set_branches", C.c_void_p(self.handle), int(mode))
 
     def path_stats(self) -> dict:
         """Per chain length: paths of the last solve with a converged item."""

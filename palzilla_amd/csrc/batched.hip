@@ -612,6 +612,12 @@ struct palz_batched {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t capture = nullptr;  // graphs are captured here (never the null stream)
+    // Captured graphs run the chain lengths as three branches: K = 2 on the
+    // capture stream, K = 1 and K = 3 on these two (forked and joined with
+    // events), so that short kernels of one length overlap the others'.
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t fork_ev[6] = {};
+    int branches = -1;  // -1: decided at capture from the chain-length mix; 0: off; 1: on
     bool timing = false;
     bool timed_solves = false;
     hipEvent_t ev[13] = {};
@@ -658,8 +664,19 @@ struct palz_batched {
         exec = nullptr;
         graph = nullptr;
     }
+    int ensure_branches() {
+        for (hipStream_t& x : side)
+            if (!x) PALZ_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+        for (hipEvent_t& e : fork_ev)
+            if (!e) PALZ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return PALZ_OK;
+    }
     ~palz_batched() {
         drop_graph();
+        for (hipStream_t x : side)
+            if (x) (void)hipStreamDestroy(x);
+        for (hipEvent_t e : fork_ev)
+            if (e) (void)hipEventDestroy(e);
         if (capture) (void)hipStreamDestroy(capture);
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -699,6 +716,48 @@ struct palz_batched {
         k_skipped<<<256, 256, 0, st>>>(ranges, L, ctr, flags, results);
         launch_exclusive_scan(flags, flags, (uint32_t)cap_items, &ctr->n_items, &ctr->converged, scan_sums, st);
         k_compact_scatter<<<cus * 8, 256, 0, st>>>(flags, ctr, ranges, L, list);
+    }
+
+    // The whole solve as three branches (see `side`): fork after the reset,
+    // join before the compaction (which scans every chain length's flags),
+    // fork again for the finishing kernels and join at the end.
+    int launch_branches(const palz_scene_desc* scenes, uint32_t n_scenes, uint32_t L, int refine,
+                        palz_path_result* results, double* refined, hipStream_t st) {
+        using namespace palz;
+        auto fork = [&](hipEvent_t e) -> int {
+            PALZ_HIP(hipEventRecord(e, st));
+            for (hipStream_t x : side) PALZ_HIP(hipStreamWaitEvent(x, e, 0));
+            return PALZ_OK;
+        };
+        auto join = [&](hipEvent_t e0, hipEvent_t e1) -> int {
+            PALZ_HIP(hipEventRecord(e0, side[0]));
+            PALZ_HIP(hipEventRecord(e1, side[1]));
+            PALZ_HIP(hipStreamWaitEvent(st, e0, 0));
+            PALZ_HIP(hipStreamWaitEvent(st, e1, 0));
+            return PALZ_OK;
+        };
+        k_reset<<<1, 64, 0, st>>>(ctr, ranges, L);
+        if (cull) launch_cull_paths(scenes, n_scenes, paths, cap_paths, ranges + 5, admissible, st);
+        int rc = fork(fork_ev[0]);
+        if (rc != PALZ_OK) return rc;
+        newton<1>(cfg[0], scenes, n_scenes, L, results, side[0]);
+        newton<3>(cfg[2], scenes, n_scenes, L, results, side[1]);
+        newton<2>(cfg[1], scenes, n_scenes, L, results, st);
+        if ((rc = join(fork_ev[1], fork_ev[2])) != PALZ_OK) return rc;
+        compact(L, results, st);
+        if ((rc = fork(fork_ev[3])) != PALZ_OK) return rc;
+        occlude<1>(cfg[0], scenes, L, side[0]);
+        finish<1>(cfg[0], scenes, L, refine, results, refined, side[0]);
+        refine_vertices<1>(cfg[0], scenes, L, refine, results, refined, side[0]);
+        occlude<3>(cfg[2], scenes, L, side[1]);
+        finish<3>(cfg[2], scenes, L, refine, results, refined, side[1]);
+        refine_vertices<3>(cfg[2], scenes, L, refine, results, refined, side[1]);
+        occlude<2>(cfg[1], scenes, L, st);
+        finish<2>(cfg[1], scenes, L, refine, results, refined, st);
+        refine_vertices<2>(cfg[1], scenes, L, refine, results, refined, st);
+        if ((rc = join(fork_ev[4], fork_ev[5])) != PALZ_OK) return rc;
+        PALZ_LAUNCH_CHECK();
+        return PALZ_OK;
     }
 
     int launch_all(const palz_scene_desc* scenes, uint32_t n_scenes, uint32_t L, int refine,
@@ -858,8 +917,23 @@ int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, u
     const bool same = b->exec && std::memcmp(&k, &b->key, sizeof(k)) == 0;
     if (!same) {
         b->drop_graph();
+        // Branches pay when no chain length dominates (many scenes of mixed
+        // lengths); when one length holds most paths its kernels fill the GPU
+        // and the others only compete with them.
+        bool branch = b->branches == 1;
+        if (b->branches < 0) {
+            uint32_t r[6];
+            PALZ_HIP(hipMemcpy(r, b->ranges, sizeof(r), hipMemcpyDeviceToHost));
+            const uint32_t most = std::max(r[1] - r[0], std::max(r[2] - r[1], r[3] - r[2]));
+            branch = (uint64_t)most * 4u < (uint64_t)(r[3] - r[0]) * 3u;
+        }
+        if (branch) {
+            rc = b->ensure_branches();
+            if (rc != PALZ_OK) return rc;
+        }
         PALZ_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
-        rc = b->launch_all(d_scenes, n_scenes, n_lambdas, refine, d_results, d_refined, b->capture, false);
+        rc = branch ? b->launch_branches(d_scenes, n_scenes, n_lambdas, refine, d_results, d_refined, b->capture)
+                    : b->launch_all(d_scenes, n_scenes, n_lambdas, refine, d_results, d_refined, b->capture, false);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(b->capture, &g);
         if (rc != PALZ_OK) return rc;
@@ -869,6 +943,14 @@ int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, u
         b->key = k;
     }
     PALZ_HIP(hipGraphLaunch(b->exec, st));
+    return PALZ_OK;
+}
+
+int palz_batched_set_branches(palz_batched* b, int mode) {
+    if (!b || mode < -1 || mode > 1)
+        return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_set_branches: mode must be -1, 0 or 1");
+    b->branches = mode;
+    b->drop_graph();
     return PALZ_OK;
 }
 

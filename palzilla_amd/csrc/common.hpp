@@ -22,6 +22,7 @@ namespace palz {
 
 static_assert(sizeof(palz_float3) == 12, "float3 layout");
 static_assert(sizeof(palz_triangle) == 36, "Triangle layout (minimum_lbvh.h:207)");
+static_assert(sizeof(palz_hit) == 28, "Hit fields (minimum_lbvh.h:905-911)");
 static_assert(sizeof(palz_aabb) == 24, "AABB layout (minimum_lbvh.h:220)");
 static_assert(sizeof(palz_internal_node) == 64, "InternalNode layout (minimum_lbvh.h:285)");
 static_assert(offsetof(palz_internal_node, aabbs) == 8, "InternalNode layout");

@@ -576,6 +576,65 @@ __host__ __device__ inline bool closest_hit(Hit* hit, const palz_internal_node* 
     return true;
 }
 
+// intersect_stackfree (minimum_lbvh.h:996-1076): the parent-pointer walk with
+// no stack.  At each internal node both child boxes are tested against the
+// current hit distance and ordered near-first; the node it came from decides
+// where to go next (down to the near child, across to the far one, or up to
+// the parent).  `any`: stop at the first triangle hit (RAY_QUERY_ANY).  A
+// leaf root (one-triangle scene) is tested directly.
+__host__ __device__ inline void closest_hit_stackfree(Hit* hit, const palz_internal_node* nodes,
+                                                      const palz_triangle* tris, uint32_t root, v3 ro, v3 rd,
+                                                      v3 ird, bool any) {
+    uint32_t curr = root, prev = PALZ_NODE_INVALID;
+    while (curr != PALZ_NODE_INVALID) {
+        if (curr & PALZ_NODE_LEAF_BIT) {
+            const uint32_t ti = curr & ~PALZ_NODE_LEAF_BIT;
+            const palz_triangle tr = ld_tri(tris, ti);
+            float t, u, v;
+            v3 ng;
+            if (intersect_tri(&t, &u, &v, &ng, 0.0f, hit->t, ro, rd, mk(tr.vs[0]), mk(tr.vs[1]), mk(tr.vs[2]))) {
+                hit->t = t;
+                hit->u = u;
+                hit->v = v;
+                hit->ng = ng;
+                hit->tri = ti;
+                if (any) break;
+            }
+            if (prev == PALZ_NODE_INVALID) break;  // the root was a leaf
+            const uint32_t x = curr;
+            curr = prev;
+            prev = x;
+            continue;
+        }
+        const palz_internal_node& nd = nodes[curr];
+        float l0, l1, r0, r1;
+        slabs(ro, ird, mk(nd.aabbs[0].lower), mk(nd.aabbs[0].upper), hit->t, &l0, &l1);
+        slabs(ro, ird, mk(nd.aabbs[1].lower), mk(nd.aabbs[1].upper), hit->t, &r0, &r1);
+        const bool hit_l = l0 <= l1, hit_r = r0 <= r1;
+        uint32_t near_node = nd.children[0], far_node = nd.children[1];
+        int n_hits = 0;
+        if (hit_l && hit_r) {
+            if (r0 < l0) {
+                near_node = nd.children[1];
+                far_node = nd.children[0];
+            }
+            n_hits = 2;
+        } else if (hit_l || hit_r) {
+            n_hits = 1;
+            near_node = hit_r ? far_node : near_node;
+        }
+        uint32_t next;
+        if (prev == nd.parent)
+            next = 0 < n_hits ? near_node : nd.parent;
+        else if (prev == near_node)
+            next = n_hits == 2 ? far_node : nd.parent;
+        else
+            next = nd.parent;
+        prev = curr;
+        curr = next;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Sobol photons (sobol.h:17-129)
 PALZ_HD uint32_t reverse_bits(uint32_t v) { return __builtin_bitreverse32(v); }

@@ -38,3 +38,8 @@ MATERIAL_DIELECTRIC = 2
 
 BUILD_OPTION_USE_NORMAL = 1
 BUILD_OPTION_CPU_PARALLEL = 2
+
+# palz_hit (minimum_lbvh::Hit, minimum_lbvh.h:905-911)
+HIT = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("ng", "<f4", 3), ("tri", "<u4")])
+assert HIT.itemsize == 28
+QUERY_CLOSEST, QUERY_CLOSEST_STACKFREE, QUERY_ANY = 0, 1, 2

@@ -270,6 +270,22 @@ def test_graph_replays_match_eager(run):
     assert res.tobytes() == res32.tobytes()
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_graph_branches_match_eager(run, mode):
+    # the captured solve as one stream or as three forked branches (one per
+    # chain length): every replay equals the eager solve
+    eager, eager_ref, eager_img = run.solve(refine=CA.REFINE_VERTICES)
+    run.bs.set_branches(mode)
+    try:
+        for rep in range(3):
+            res, ref, img = run.solve(refine=CA.REFINE_VERTICES, graph=True)
+            assert res.tobytes() == eager.tobytes(), f"branches={mode}, replay {rep}"
+            assert ref.tobytes() == eager_ref.tobytes(), f"branches={mode}, replay {rep}"
+            assert np.allclose(img, eager_img, rtol=1e-5, atol=1e-6), f"branches={mode}, replay {rep}"
+    finally:
+        run.bs.set_branches(-1)
+
+
 def test_autotune_keeps_results(run):
     before, _, _ = run.solve(refine=CA.REFINE_VERTICES)
     rep = run.bs.autotune(run.d_scenes, 1, run.L, refine=CA.REFINE_VERTICES)

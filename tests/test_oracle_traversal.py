@@ -146,3 +146,31 @@ def test_ray_triangle_agrees_with_float64_moller_trumbore():
                 # one of the edge opposite its vertex (minimum_lbvh.h:383-397)
                 assert abs(got[1] - u) < 1e-4 and abs(got[2] - vv) < 1e-4
     assert n_checked > 2000 and 0.1 * n_checked < n_hit < 0.9 * n_checked
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_stackfree_walk_equals_the_stackful_walk_and_brute_force(name):
+    # intersect_stackfree (minimum_lbvh.h:996-1076), the parent-pointer walk:
+    # the same nearest t as intersect_stackfull and as the brute force; its
+    # any-hit form finds a hit over [0, 1] exactly when the brute force does
+    tris, rng = CASES[name]
+    nodes, root = _bvh(tris)
+    ro, rd = _rays_into(tris, 1500, rng)
+    full = oracle.intersect_rays(tris, nodes, root, ro, rd, 0)
+    free = oracle.intersect_rays(tris, nodes, root, ro, rd, 1)
+    bt, btri, _ = oracle.brute_hits(tris, ro, rd)
+    assert np.array_equal(free["t"].view(np.uint32), bt.view(np.uint32))
+    assert np.array_equal(free["t"].view(np.uint32), full["t"].view(np.uint32))
+    same = free["tri"] == full["tri"]
+    assert same.mean() > 0.99
+    for i in np.nonzero(~same)[0]:  # exact ties only
+        r = oracle.ray_triangle(tris[free["tri"][i]], ro[i], rd[i])
+        assert r is not None and r[0] == bt[i]
+    seg = rd * rng.uniform(0.3, 1.3, size=(len(rd), 1)).astype(np.float32)
+    anyh = oracle.intersect_rays(tris, nodes, root, ro, seg, 2, 1.0)
+    _, _, want = oracle.brute_hits(tris, ro, seg, 1.0)
+    assert np.array_equal(anyh["tri"] != 0xFFFFFFFF, want)
+    # an any-hit reports a real hit on that triangle within the range
+    for i in np.nonzero(want)[0][:200]:
+        r = oracle.ray_triangle(tris[anyh["tri"][i]], ro[i], seg[i], 0.0, 1.0)
+        assert r is not None and r[0] == anyh["t"][i]
