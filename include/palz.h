@@ -465,6 +465,17 @@ int palz_solve_specular(int K, uint32_t event, const palz_scene_desc* scene, con
  * naming a scene >= n_scenes).  `d_count` (optional, device) caps n_paths. */
 int palz_interval_cull(const palz_scene_desc* d_scenes, uint32_t n_scenes, const palz_path* d_paths,
                        uint32_t n_paths, const uint32_t* d_count, uint8_t* d_admissible, void* stream);
+/* The same test in affine arithmetic, after the interval test (no
+ * counterpart in the reference, which only sketches affine forms as an
+ * idea): each vertex and interpolated shading normal is an affine form in
+ * its triangle's two barycentric parameters, and eta one in its own noise
+ * symbol, so wi and wo of a vertex keep the vertex they share; coefficients
+ * round to nearest with their exact errors (TwoSum, FMA residuals) added
+ * outward, square root and reciprocal by linear bounds with directed
+ * rounding.  d_admissible[p] = 0 when either test shows the path has no
+ * solution; same arguments and guarantees as palz_interval_cull. */
+int palz_affine_cull(const palz_scene_desc* d_scenes, uint32_t n_scenes, const palz_path* d_paths, uint32_t n_paths,
+                     const uint32_t* d_count, uint8_t* d_admissible, void* stream);
 
 /* Node-level path cuts: traverseAdmissibleNodes<K> (pk.h:1760-2212), called
  * by the reference from host code (main.cpp:930-944, :976-1070), here one
@@ -646,16 +657,19 @@ int palz_batched_path_stats(palz_batched* b, uint64_t* converged_paths3, void* s
  * its kernels; palz_batched_timings waits for the last solve's events and
  * returns 12 durations in ms: Newton K=1, Newton K=2, Newton K=3, the
  * compaction of the converged items, then for K = 1, 2, 3 the occlusion rays
- * (contributableThroughput) and the finish (dA/dw, irradiance, results), the
- * FP64 refinement of all three (PALZ_REFINE_VERTICES; else ~0) and the
- * interval culling (when enabled; else ~0). */
+ * (contributableThroughput) and the finish (dA/dw, irradiance, results and,
+ * with refinement on, the FP64 refinement step), then a refinement slot that
+ * stays ~0 (the step runs inside the finish kernels) and the interval culling
+ * (when enabled; else ~0). */
 int palz_batched_set_timing(palz_batched* b, int enable);
 int palz_batched_timings(palz_batched* b, float* ms12);
-/* Per-tuple interval culling (palz_interval_cull) in front of the Newton
- * kernels: items of a culled path get status PALZ_PATH_CULLED without a
- * solve.  A culled path has no contributing solution, so the image and the
- * contributing items are those of an unculled solve.  Off by default. */
-int palz_batched_set_culling(palz_batched* b, int enable);
+/* Per-tuple culling in front of the Newton kernels: items of a culled path
+ * get status PALZ_PATH_CULLED without a solve.  mode 1: the interval test
+ * (palz_interval_cull); 2: the interval test, then the affine test
+ * (palz_affine_cull) on the paths it keeps; 0 (default): off.  A culled path
+ * has no contributing solution, so the image and the contributing items are
+ * those of an unculled solve. */
+int palz_batched_set_culling(palz_batched* b, int mode);
 /* Time every candidate launch shape per chain length on the prepared batch
  * and keep the fastest (the autotuner).  The image receives the candidates'
  * contributions; clear it afterwards.  Writes the chosen shapes and their

@@ -51,6 +51,7 @@ _SIGS = {
     "orc_solve_primary": (None, [_P, _P, _P, C.c_uint32, _P, _P, _P, _P, _P]),
     "orc_occlusion_work": (None, [_P, _P, C.c_uint32, _P]),
     "orc_fixed_word": (C.c_uint64, [C.c_float]),
+    "orc_refine_step": (C.c_int, [_P, _P, _P, _P, _P, C.c_float, _P, _P]),
     "orc_fresnel": (C.c_float, [_P, _P, C.c_float]),
     "orc_refraction": (C.c_int, [_P, _P, C.c_float, _P]),
     "orc_dual_optics": (None, [C.c_int, _P, _P, C.c_float, _P]),
@@ -313,6 +314,22 @@ def solve_f64(scene, path, p_end, eta: float, params0, tol: float = 1e-28, max_i
     ok = lib().orc_solve_f64(_ptr(scene.tris), _ptr(scene.attribs), _ptr(p), _ptr(pl), _ptr(pe), float(eta), tol,
                              max_iter, _ptr(prm), _ptr(verts))
     return bool(ok), prm[:2 * K], verts[:3 * K].reshape(K, 3)
+
+
+def refine_step(scene, path, p_end, eta: float, params32) -> np.ndarray:
+    """The batched solver's refinement step from the FP32 solution params32:
+    FP64 residual, FP32 Jacobian, FP64 least-squares correction (2K doubles)."""
+    K = int(path["event"]) >> 24
+    p = np.ascontiguousarray(np.array([path]))
+    pl = np.asarray(scene.p_light, dtype=np.float32)
+    pe = np.asarray(p_end, dtype=np.float32)
+    prm = np.zeros(8, np.float32)
+    prm[:2 * K] = np.asarray(params32, np.float32)[:2 * K]
+    out = np.zeros(8, np.float64)
+    if not lib().orc_refine_step(_ptr(scene.tris), _ptr(scene.attribs), _ptr(p), _ptr(pl), _ptr(pe), float(eta),
+                                 _ptr(prm), _ptr(out)):
+        raise ValueError("chain length must be 1..3")
+    return out[:2 * K]
 
 
 def count_flops(scene, path, p_end, eta: float):

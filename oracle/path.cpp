@@ -257,6 +257,50 @@ struct Scene {
 
 bool is_T(uint32_t event, int k) { return (event >> k) & 1u; }
 
+// The residual b, its Jacobian A and the cost of one solveConstraints
+// iteration (pk.h:479-548): 2K single-tangent passes.
+template <int K, class T>
+void evalConstraints(const T* parameters, vec3<T> p_beg, vec3<T> p_end, const palz_triangle* tris,
+                     const palz_triangle_attrib* attribs, T eta, uint32_t event, Mat<T, K * 3, K * 2>& A,
+                     Mat<T, K * 3, 1>& b, T* cost_out) {
+    const int nParameters = K * 2;
+    T cost = T(0);
+    for (int i = 0; i < nParameters; i++) {
+        dval3<T> vertices[K + 2];
+        vertices[0] = mkd(p_beg);
+        vertices[K + 1] = mkd(p_end);
+        dval3<T> shadingNormals[K];
+        for (int k = 0; k < K; k++) {
+            dval<T> param_u(parameters[k * 2 + 0], k * 2 + 0 == i);
+            dval<T> param_v(parameters[k * 2 + 1], k * 2 + 1 == i);
+            // edges in the working precision (for T = float: the reference's float3 ops)
+            vec3<T> v0 = cvt<T>(tris[k].vs[0]), v1 = cvt<T>(tris[k].vs[1]), v2 = cvt<T>(tris[k].vs[2]);
+            vec3<T> e0 = v1 - v0, e1 = v2 - v0;
+            vertices[k + 1] = mkd(v0) + mkd(e0) * param_u + mkd(e1) * param_v;
+            vec3<T> n0 = cvt<T>(attribs[k].shading_normals[0]);
+            vec3<T> ne0 = cvt<T>(attribs[k].shading_normals[1]) - n0;
+            vec3<T> ne1 = cvt<T>(attribs[k].shading_normals[2]) - n0;
+            shadingNormals[k] = mkd(n0) + mkd(ne0) * param_u + mkd(ne1) * param_v;
+        }
+        for (int k = 0; k < K; k++) {
+            dval3<T> wi = vertices[k] - vertices[k + 1];
+            dval3<T> wo = vertices[k + 2] - vertices[k + 1];
+            dval3<T> n = shadingNormals[k];
+            if (dot(wi, n).v < T(0)) std::swap(wi, wo);
+            dval3<T> wo_optimize = is_T(event, k) ? refraction_norm_free(wi, n, eta) : reflection(wi, n);
+            dval3<T> c = cross(wo, wo_optimize);
+            A(k * 3 + 0, i) = c.x.g;
+            A(k * 3 + 1, i) = c.y.g;
+            A(k * 3 + 2, i) = c.z.g;
+            b(k * 3 + 0, 0) = c.x.v;
+            b(k * 3 + 1, 0) = c.y.v;
+            b(k * 3 + 2, 0) = c.z.v;
+            if (i == 0) cost += dot(c, c).v;
+        }
+    }
+    *cost_out = cost;
+}
+
 // solveConstraints<K> (pk.h:466-592)
 template <int K, class T>
 bool solveConstraints(T* parameters, vec3<T> p_beg, vec3<T> p_end, const palz_triangle* tris,
@@ -268,39 +312,7 @@ bool solveConstraints(T* parameters, vec3<T> p_beg, vec3<T> p_end, const palz_tr
         Mat<T, K * 3, K * 2> A;
         Mat<T, K * 3, 1> b;
         T cost = T(0);
-        for (int i = 0; i < nParameters; i++) {
-            dval3<T> vertices[K + 2];
-            vertices[0] = mkd(p_beg);
-            vertices[K + 1] = mkd(p_end);
-            dval3<T> shadingNormals[K];
-            for (int k = 0; k < K; k++) {
-                dval<T> param_u(parameters[k * 2 + 0], k * 2 + 0 == i);
-                dval<T> param_v(parameters[k * 2 + 1], k * 2 + 1 == i);
-                // edges in the working precision (for T = float: the reference's float3 ops)
-                vec3<T> v0 = cvt<T>(tris[k].vs[0]), v1 = cvt<T>(tris[k].vs[1]), v2 = cvt<T>(tris[k].vs[2]);
-                vec3<T> e0 = v1 - v0, e1 = v2 - v0;
-                vertices[k + 1] = mkd(v0) + mkd(e0) * param_u + mkd(e1) * param_v;
-                vec3<T> n0 = cvt<T>(attribs[k].shading_normals[0]);
-                vec3<T> ne0 = cvt<T>(attribs[k].shading_normals[1]) - n0;
-                vec3<T> ne1 = cvt<T>(attribs[k].shading_normals[2]) - n0;
-                shadingNormals[k] = mkd(n0) + mkd(ne0) * param_u + mkd(ne1) * param_v;
-            }
-            for (int k = 0; k < K; k++) {
-                dval3<T> wi = vertices[k] - vertices[k + 1];
-                dval3<T> wo = vertices[k + 2] - vertices[k + 1];
-                dval3<T> n = shadingNormals[k];
-                if (dot(wi, n).v < T(0)) std::swap(wi, wo);
-                dval3<T> wo_optimize = is_T(event, k) ? refraction_norm_free(wi, n, eta) : reflection(wi, n);
-                dval3<T> c = cross(wo, wo_optimize);
-                A(k * 3 + 0, i) = c.x.g;
-                A(k * 3 + 1, i) = c.y.g;
-                A(k * 3 + 2, i) = c.z.g;
-                b(k * 3 + 0, 0) = c.x.v;
-                b(k * 3 + 1, 0) = c.y.v;
-                b(k * 3 + 2, 0) = c.z.v;
-                if (i == 0) cost += dot(c, c).v;
-            }
-        }
+        evalConstraints<K, T>(parameters, p_beg, p_end, tris, attribs, eta, event, A, b, &cost);
         if (cost < costTolerance) {
             *iters = iter;
             return true;
@@ -321,6 +333,31 @@ bool solveConstraints(T* parameters, vec3<T> p_beg, vec3<T> p_end, const palz_tr
     }
     *iters = maxIterations;
     return false;
+}
+
+// The refinement step of the batched solver: one Newton step from the FP32
+// solution with the residual evaluated in FP64 at the FP32 parameters, the
+// Jacobian of the FP32 iteration at the same point (float dual passes,
+// entries widened to double) and the least-squares correction in FP64 (mixed
+// precision iterative refinement: the residual carries the precision).
+template <int K>
+void refineStep(const float* params32, float3 p_beg, float3 p_end, const palz_triangle* tris,
+                const palz_triangle_attrib* attribs, float eta, uint32_t event, double* out) {
+    Mat<float, K * 3, K * 2> A32;
+    Mat<float, K * 3, 1> b32;
+    float cost32;
+    evalConstraints<K, float>(params32, p_beg, p_end, tris, attribs, eta, event, A32, b32, &cost32);
+    double p64[K * 2];
+    for (int i = 0; i < 2 * K; i++) p64[i] = (double)params32[i];
+    Mat<double, K * 3, K * 2> A;
+    Mat<double, K * 3, 1> b;
+    double cost;
+    evalConstraints<K, double>(p64, cvt<double>(p_beg), cvt<double>(p_end), tris, attribs, (double)eta, event, A,
+                               b, &cost);
+    for (int i = 0; i < K * 3 * K * 2; i++) A.m[i] = (double)A32.m[i];
+    double dp[K * 2];
+    solve_qr_overdetermined(A, b, dp);
+    for (int i = 0; i < 2 * K; i++) out[i] = p64[i] - dp[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1038,6 +1075,27 @@ int orc_solve_f64(const palz_triangle* tris_all, const palz_triangle_attrib* att
         vertices_out[3 * k + 2] = x.z;
     }
     return ok ? 1 : 0;
+}
+
+// The batched solver's refinement step (refineStep above) of one chain from
+// its FP32 solution params32; writes 2K doubles.  Returns 0 for K outside 1..3.
+int orc_refine_step(const palz_triangle* tris_all, const palz_triangle_attrib* attribs_all, const palz_path* path,
+                    const float* p_light, const float* p_end, float eta, const float* params32, double* out) {
+    const int K = (int)(path->event >> 24);
+    palz_triangle tris[3];
+    palz_triangle_attrib attribs[3];
+    if (K < 1 || K > 3) return 0;
+    for (int k = 0; k < K; k++) {
+        tris[k] = tris_all[path->tris[k]];
+        attribs[k] = attribs_all[path->tris[k]];
+    }
+    const float3 pb = {p_light[0], p_light[1], p_light[2]}, pe = {p_end[0], p_end[1], p_end[2]};
+    switch (K) {
+        case 1: refineStep<1>(params32, pb, pe, tris, attribs, eta, path->event, out); break;
+        case 2: refineStep<2>(params32, pb, pe, tris, attribs, eta, path->event, out); break;
+        default: refineStep<3>(params32, pb, pe, tris, attribs, eta, path->event, out); break;
+    }
+    return 1;
 }
 
 // Arithmetic of one solveConstraints iteration (pk.h:477-590) on this chain,

@@ -273,20 +273,29 @@ __device__ __forceinline__ uint32_t compact_pos(const uint32_t* excl, uint32_t j
     return j < n ? excl[j] : total;
 }
 
-// One Newton step in FP64 from the FP32 solution (two tangents per pass).
+// One refinement step from the FP32 solution, mixed-precision iterative
+// refinement: the residual evaluated in FP64 at the FP32 parameters (the
+// precision lives there), the Jacobian of the FP32 iteration at the same
+// point (one multi-tangent FP32 pass, widened to double) and the
+// least-squares correction solved in FP64.  The FP32 Jacobian's relative
+// error (~1e-7) scales a correction that is itself ~1e-5 of the parameters.
 template <int K>
 __device__ __forceinline__ void refine_step(const float* prm, v3 pb, v3 pe, const ChainTri* tri, float eta,
                                             uint32_t event, double* p64) {
     constexpr int P = 2 * K;
     constexpr int R = 3 * K;
+    float A32[R * P];
+    float b32[R];
+    float cost32 = 0.0f;
+    chain_residual<K, P, float, v3>(prm, 0, pb, pe, tri, eta, event, A32, b32, &cost32);
 #pragma unroll
     for (int i = 0; i < P; ++i) p64[i] = (double)prm[i];
     double A[R * P];
     double b[R];
     double cost = 0.0;
+    chain_residual<K, 0, double, d3>(p64, 0, mkd(pb), mkd(pe), tri, (double)eta, event, A, b, &cost);
 #pragma unroll
-    for (int first = 0; first < P; first += 2)
-        chain_residual<K, 2, double, d3>(p64, first, mkd(pb), mkd(pe), tri, (double)eta, event, A, b, &cost);
+    for (int i = 0; i < R * P; ++i) A[i] = (double)A32[i];
     double dp[P];
     solve_qr<R, P, double>(A, b, dp);
 #pragma unroll
@@ -399,9 +408,11 @@ __global__ void __launch_bounds__(256, 8)  // 8 waves per SIMD: latency-bound, f
 }
 
 // dA/dw and irradiance of the converged chains of length K, in item order,
-// from the throughput k_occlude left per list position.  kRefined: the FP64 step runs here and its chain feeds dA/dw
-// (PALZ_REFINE_ALL); otherwise dA/dw uses the FP32 chain.
-template <int K, bool kRefined>
+// from the throughput k_occlude left per list position.  kMode: the refinement
+// mode.  PALZ_REFINE_VERTICES and PALZ_REFINE_ALL run the FP64 step here on
+// the item's data already in registers; with ALL its chain feeds dA/dw,
+// otherwise dA/dw uses the FP32 chain.
+template <int K, int kMode>
 __global__ void __launch_bounds__(256)
     k_finish(const palz_scene_desc* __restrict__ scenes, const palz_path* __restrict__ paths,
              const uint32_t* __restrict__ ranges, uint32_t L, Counters* __restrict__ ctr,
@@ -431,14 +442,19 @@ __global__ void __launch_bounds__(256)
         const float eta = item_eta(sc, l);
         const uint32_t event = path.event;  // per lane (see k_solve_plain)
         const float thr = thr_in[s];        // contributableThroughput, from k_occlude
-        float pf[P];
+        float pf[P];  // the reported parameters
+        float pd[P];  // the chain dA/dw and the receiver cosine take
 #pragma unroll
-        for (int i = 0; i < P; ++i) pf[i] = prm[i];
-        if (kRefined) {
+        for (int i = 0; i < P; ++i) pf[i] = pd[i] = prm[i];
+        if (kMode != PALZ_REFINE_NONE) {
             double p64[P];
             refine_step<K>(prm, pb, pe, tri, eta, event, p64);
 #pragma unroll
             for (int i = 0; i < P; ++i) pf[i] = (float)p64[i];
+            if (kMode == PALZ_REFINE_ALL) {
+#pragma unroll
+                for (int i = 0; i < P; ++i) pd[i] = pf[i];
+            }
             if (refined) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) refined[(size_t)item * 8 + i] = i < P ? p64[i] : 0.0;
@@ -452,8 +468,8 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int i = 0; i < 8; ++i) r.params[i] = i < P ? pf[i] : 0.0f;
         if (0.0f < thr) {
-            const float area = dadw<K>(pb, chain_vertex(tri[0], pf[0], pf[1]) - pb, pe, tri, event, eta);
-            const v3 last = chain_vertex(tri[K - 1], pf[P - 2], pf[P - 1]);
+            const float area = dadw<K>(pb, chain_vertex(tri[0], pd[0], pd[1]) - pb, pe, tri, event, eta);
+            const v3 last = chain_vertex(tri[K - 1], pd[P - 2], pd[P - 1]);
             float c = thr * sc->light_intensity / area * hmax(dot(normalize(last - pe), mk(q.ng)), 0.0f);
             c = hmin(c, sc->radiance_clamp);
             add_contribution(*sc, (size_t)l * sc->n_points + path.pixel, c);
@@ -465,44 +481,6 @@ __global__ void __launch_bounds__(256)
         if (results) results[item] = r;
     }
     wave_add_stat(&ctr->stats[(K - 1) * 4 + 2], n_contrib);
-}
-
-// PALZ_REFINE_VERTICES: the FP64 step of every converged chain of length K,
-// after the finish kernel (its irradiance stays on the FP32 chain); writes the
-// refined barycentrics into the results and the FP64 output.
-template <int K>
-__global__ void __launch_bounds__(256)
-    k_refine(const palz_scene_desc* __restrict__ scenes, const palz_path* __restrict__ paths,
-             const uint32_t* __restrict__ ranges, uint32_t L, const Counters* __restrict__ ctr,
-             const uint32_t* __restrict__ excl, const uint32_t* __restrict__ list,
-             const float* __restrict__ conv_params, palz_path_result* __restrict__ results,
-             double* __restrict__ refined) {
-    constexpr int P = 2 * K;
-    const uint32_t n_items = ctr->n_items, total = ctr->converged;
-    const uint32_t begin = compact_pos(excl, ranges[K - 1] * L, n_items, total);
-    const uint32_t end = compact_pos(excl, ranges[K] * L, n_items, total);
-    for (uint32_t s = begin + blockIdx.x * blockDim.x + threadIdx.x; s < end; s += gridDim.x * blockDim.x) {
-        const uint32_t item = list[s];
-        const uint32_t pi = item / L, l = item % L;
-        const palz_path path = paths[pi];
-        const palz_scene_desc* sc = scenes + path.scene;
-        ChainTri tri[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) tri[k] = load_chain_tri(sc->tris, sc->attribs, path.tris[k]);
-        float prm[P];
-#pragma unroll
-        for (int i = 0; i < P; ++i) prm[i] = conv_params[(size_t)item * kQueueStride + i];
-        double p64[P];
-        refine_step<K>(prm, mk(sc->p_light), mk(sc->points[path.pixel].p), tri, item_eta(sc, l), path.event, p64);
-        if (results) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) results[item].params[i] = i < P ? (float)p64[i] : 0.0f;
-        }
-        if (refined) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) refined[(size_t)item * 8 + i] = i < P ? p64[i] : 0.0;
-        }
-    }
 }
 
 // Items of chain length 4 (and of none) are not part of a batch: mark and
@@ -570,19 +548,15 @@ void launch_finish(const palz_launch_config& c, int cus, hipStream_t st, const p
                    const uint32_t* list, const float* cp, const int* cit, const float* thr, int refine,
                    palz_path_result* results, double* refined) {
     const dim3 grid(cus * 8), block(c.finish_block);
-    if (refine == PALZ_REFINE_ALL) {
-        k_finish<K, true><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, cit, thr, results,
-                                                  refined);
-        return;
-    }
-    k_finish<K, false><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, cit, thr, results,
-                                               nullptr);
-}
-template <int K>
-void launch_refine(const palz_launch_config& c, int cus, hipStream_t st, const palz_scene_desc* scenes,
-                   const palz_path* paths, const uint32_t* ranges, uint32_t L, Counters* ctr, const uint32_t* excl,
-                   const uint32_t* list, const float* cp, palz_path_result* results, double* refined) {
-    k_refine<K><<<cus * 8, c.finish_block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, results, refined);
+    if (refine == PALZ_REFINE_ALL)
+        k_finish<K, PALZ_REFINE_ALL><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, cit, thr,
+                                                             results, refined);
+    else if (refine == PALZ_REFINE_VERTICES)
+        k_finish<K, PALZ_REFINE_VERTICES><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, cit,
+                                                                  thr, results, refined);
+    else
+        k_finish<K, PALZ_REFINE_NONE><<<grid, block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, cit, thr,
+                                                              results, nullptr);
 }
 
 }  // namespace
@@ -621,7 +595,7 @@ struct palz_batched {
     bool timing = false;
     bool timed_solves = false;
     hipEvent_t ev[13] = {};
-    bool cull = false;
+    int cull = 0;  // 0 off, 1 interval test, 2 interval then affine test
     uint8_t* admissible = nullptr;  // per sorted path, when culling
     struct Key {
         const void *scenes, *results, *refined;
@@ -703,13 +677,6 @@ struct palz_batched {
         palz::launch_finish<K>(c, cus, st, scenes, paths, ranges, L, ctr, flags, list, conv_params, conv_iters, thr,
                                refine, results, refined);
     }
-    template <int K>
-    void refine_vertices(const palz_launch_config& c, const palz_scene_desc* scenes, uint32_t L, int refine,
-                         palz_path_result* results, double* refined, hipStream_t st) {
-        if (refine == PALZ_REFINE_VERTICES)
-            palz::launch_refine<K>(c, cus, st, scenes, paths, ranges, L, ctr, flags, list, conv_params, results,
-                                   refined);
-    }
     // K = 4 marks, the scan of the converged flags and the scatter into the list
     void compact(uint32_t L, palz_path_result* results, hipStream_t st) {
         using namespace palz;
@@ -737,7 +704,7 @@ struct palz_batched {
             return PALZ_OK;
         };
         k_reset<<<1, 64, 0, st>>>(ctr, ranges, L);
-        if (cull) launch_cull_paths(scenes, n_scenes, paths, cap_paths, ranges + 5, admissible, st);
+        if (cull) launch_cull_paths(scenes, n_scenes, paths, cap_paths, ranges + 5, admissible, st, cull);
         int rc = fork(fork_ev[0]);
         if (rc != PALZ_OK) return rc;
         newton<1>(cfg[0], scenes, n_scenes, L, results, side[0]);
@@ -748,13 +715,10 @@ struct palz_batched {
         if ((rc = fork(fork_ev[3])) != PALZ_OK) return rc;
         occlude<1>(cfg[0], scenes, L, side[0]);
         finish<1>(cfg[0], scenes, L, refine, results, refined, side[0]);
-        refine_vertices<1>(cfg[0], scenes, L, refine, results, refined, side[0]);
         occlude<3>(cfg[2], scenes, L, side[1]);
         finish<3>(cfg[2], scenes, L, refine, results, refined, side[1]);
-        refine_vertices<3>(cfg[2], scenes, L, refine, results, refined, side[1]);
         occlude<2>(cfg[1], scenes, L, st);
         finish<2>(cfg[1], scenes, L, refine, results, refined, st);
-        refine_vertices<2>(cfg[1], scenes, L, refine, results, refined, st);
         if ((rc = join(fork_ev[4], fork_ev[5])) != PALZ_OK) return rc;
         PALZ_LAUNCH_CHECK();
         return PALZ_OK;
@@ -766,7 +730,7 @@ struct palz_batched {
         const bool t = timed && timing;
         k_reset<<<1, 64, 0, st>>>(ctr, ranges, L);
         if (t) PALZ_HIP(hipEventRecord(ev[0], st));
-        if (cull) launch_cull_paths(scenes, n_scenes, paths, cap_paths, ranges + 5, admissible, st);
+        if (cull) launch_cull_paths(scenes, n_scenes, paths, cap_paths, ranges + 5, admissible, st, cull);
         if (t) PALZ_HIP(hipEventRecord(ev[9], st));
         newton<1>(cfg[0], scenes, n_scenes, L, results, st);
         if (t) PALZ_HIP(hipEventRecord(ev[1], st));
@@ -788,9 +752,6 @@ struct palz_batched {
         if (t) PALZ_HIP(hipEventRecord(ev[12], st));
         finish<3>(cfg[2], scenes, L, refine, results, refined, st);
         if (t) PALZ_HIP(hipEventRecord(ev[7], st));
-        refine_vertices<1>(cfg[0], scenes, L, refine, results, refined, st);
-        refine_vertices<2>(cfg[1], scenes, L, refine, results, refined, st);
-        refine_vertices<3>(cfg[2], scenes, L, refine, results, refined, st);
         if (t) PALZ_HIP(hipEventRecord(ev[8], st));
         PALZ_LAUNCH_CHECK();
         if (t) timed_solves = true;
@@ -912,7 +873,7 @@ int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, u
     int rc = check_solve_args(b, d_scenes, n_scenes, n_lambdas, refine);
     if (rc != PALZ_OK) return rc;
     hipStream_t st = (hipStream_t)stream;
-    palz_batched::Key k{d_scenes, d_results, d_refined, n_scenes, n_lambdas, refine, {}, b->cull ? 1 : 0};
+    palz_batched::Key k{d_scenes, d_results, d_refined, n_scenes, n_lambdas, refine, {}, b->cull};
     std::copy(b->cfg, b->cfg + 3, k.cfg);
     const bool same = b->exec && std::memcmp(&k, &b->key, sizeof(k)) == 0;
     if (!same) {
@@ -978,7 +939,9 @@ int palz_batched_timings(palz_batched* b, float* ms12) {
 
 int palz_batched_set_culling(palz_batched* b, int enable) {
     if (!b) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_set_culling: NULL");
-    b->cull = enable != 0;
+    if (enable < 0 || enable > 2)
+        return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_set_culling: mode must be 0, 1 or 2");
+    b->cull = enable;
     b->drop_graph();
     return PALZ_OK;
 }
@@ -1029,7 +992,8 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
         if (K == 3) b->newton<3>(c, d_scenes, n_scenes, n_lambdas, nullptr, st);
     };
     float newton_ms[3] = {0.0f, 0.0f, 0.0f}, finish_ms[3] = {0.0f, 0.0f, 0.0f};
-    if (b->cull) launch_cull_paths(d_scenes, n_scenes, b->paths, b->cap_paths, b->ranges + 5, b->admissible, st);
+    if (b->cull)
+        launch_cull_paths(d_scenes, n_scenes, b->paths, b->cap_paths, b->ranges + 5, b->admissible, st, b->cull);
     // Newton kernels: block size, resident blocks per CU and queue chunk
     for (int K = 1; K <= 3; ++K) {
         if (ranges[K] == ranges[K - 1]) continue;
@@ -1075,9 +1039,6 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
             if (K == 1) b->finish<1>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
             if (K == 2) b->finish<2>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
             if (K == 3) b->finish<3>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
-            if (K == 1) b->refine_vertices<1>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
-            if (K == 2) b->refine_vertices<2>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
-            if (K == 3) b->refine_vertices<3>(c, d_scenes, n_lambdas, refine, nullptr, nullptr, st);
             const float ms = elapsed();
             if (ms < best_fin) {
                 best_fin = ms;
