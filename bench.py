@@ -233,18 +233,23 @@ def bench_c2(args) -> dict:
     stats = bs.stats()
     path_stats = bs.path_stats()
 
-    # the same solve with the per-tuple interval culling in front of Newton
-    bs.set_culling(True)
-    bs.set_timing(True)
-    bs.solve(d_scenes, 1, L, CA.REFINE_VERTICES, d_res)
-    cull_kern = bs.timings()
-    cull_stats = bs.stats()
-    bs.set_timing(False)
-    bs.set_culling(False)
-    culling = {"culled_items": cull_stats["culled"], "items": n_items,
-               "cull_kernel_ms": round(cull_kern["cull"], 3),
-               "step_ms_with_culling": round(sum(cull_kern.values()), 3),
-               "contributing_items_with_culling": sum(cull_stats[f"K{k}"]["contributing"] for k in (1, 2, 3))}
+    # the same solve with the per-tuple culling in front of Newton: the
+    # interval test (mode 1), then the interval test followed by the affine
+    # test (mode 2)
+    culling = {"items": n_items}
+    for mode, name in ((1, "interval"), (2, "affine")):
+        log(f"[bench] c2: one solve with {name} culling")
+        bs.set_culling(mode)
+        bs.set_timing(True)
+        bs.solve(d_scenes, 1, L, CA.REFINE_VERTICES, d_res)
+        cull_kern = bs.timings()
+        cull_stats = bs.stats()
+        bs.set_timing(False)
+        bs.set_culling(0)
+        culling[name] = {"culled_items": cull_stats["culled"], "cull_kernel_ms": round(cull_kern["cull"], 3),
+                         "step_ms_with_culling": round(sum(cull_kern.values()), 3),
+                         "contributing_items_with_culling":
+                             sum(cull_stats[f"K{k}"]["contributing"] for k in (1, 2, 3))}
 
     # spectral resolve of the receiver images (CIE fit, XYZ -> sRGB, pack):
     # reads L images and the receiver points, writes float4 + RGBA8 per point
@@ -432,7 +437,7 @@ def bench_c2(args) -> dict:
                   "batched_over_plain": round(value / plain_value, 2) if plain_value else None,
                   "batched_over_plain_time": round(ms_plain / ms, 2),
                   "image_rel_rmse_batched_vs_plain": img_rmse},
-        "interval_culling": culling,
+        "culling": culling,
         "spectral_resolve": resolve,
         "solve_primary": primary,
         "vertex_error": sample["vertex"],

@@ -365,8 +365,10 @@ class BatchedSolver:
         call("palz_batched_timings", C.c_void_p(self.handle), ms)
         return dict(zip(self.TIMED_KERNELS, (float(v) for v in ms)))
 
-    def set_culling(self, enable: bool) -> None:
-        call("palz_batched_set_culling", C.c_void_p(self.handle), int(bool(enable)))
+    def set_culling(self, mode: int | bool) -> None:
+        """0/False: off; 1/True: interval test; 2: interval test, then affine test."""
+        mode = int(mode)
+        call("palz_batched_set_culling", C.c_void_p(self.handle), mode)
 
     def stats(self) -> dict:
         s = (C.c_uint64 * 16)()
@@ -607,6 +609,16 @@ def interval_cull(d_scenes: DeviceBuffer, n_scenes: int, d_paths: DeviceBuffer |
     ptr = d_paths.ptr if isinstance(d_paths, DeviceBuffer) else d_paths
     d_out = DeviceBuffer(max(n_paths, 1))
     call("palz_interval_cull", C.c_void_p(d_scenes.ptr), C.c_uint32(n_scenes), C.c_void_p(ptr), C.c_uint32(n_paths),
+         C.c_void_p(None), C.c_void_p(d_out.ptr), _sp(stream))
+    return d_out.download(np.uint8, n_paths, stream)
+
+
+def affine_cull(d_scenes: DeviceBuffer, n_scenes: int, d_paths: DeviceBuffer | int, n_paths: int,
+                stream=None) -> np.ndarray:
+    """palz_affine_cull: the interval test, then the affine test on what it keeps (1 = keep)."""
+    ptr = d_paths.ptr if isinstance(d_paths, DeviceBuffer) else d_paths
+    d_out = DeviceBuffer(max(n_paths, 1))
+    call("palz_affine_cull", C.c_void_p(d_scenes.ptr), C.c_uint32(n_scenes), C.c_void_p(ptr), C.c_uint32(n_paths),
          C.c_void_p(None), C.c_void_p(d_out.ptr), _sp(stream))
     return d_out.download(np.uint8, n_paths, stream)
 

@@ -216,9 +216,10 @@ constexpr int kSplitBudget = 64;  // rectangles per chain
 // any combination of rectangles passes every vertex.  False only if no point
 // of the triangles can bring every vertex's residual within the solver's
 // tolerance.
+// The split of each chain triangle's parameter square (a power of two per
+// axis, at most kMaxSplit, within kSplitBudget rectangles in all).
 template <int K>
-PALZ_HD bool chain_admissible(v3 p_beg, v3 p_end, const ChainTri* tri, uint32_t event, I eta) {
-    int split[K];
+PALZ_HD void chain_splits(const ChainTri* tri, int* split) {
     float ext[K], ext_min = kInf;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -242,6 +243,15 @@ PALZ_HD bool chain_admissible(v3 p_beg, v3 p_end, const ChainTri* tri, uint32_t 
         combos /= 4;
         split[kk] /= 2;
     }
+}
+
+template <int K>
+PALZ_HD bool chain_admissible(v3 p_beg, v3 p_end, const ChainTri* tri, uint32_t event, I eta) {
+    int split[K];
+    chain_splits<K>(tri, split);
+    int combos = 1;
+#pragma unroll
+    for (int k = 0; k < K; ++k) combos *= split[k] * split[k];
     for (int c = 0; c < combos; ++c) {
         I3 vert[K + 2];
         I3 nrm[K];

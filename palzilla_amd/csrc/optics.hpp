@@ -240,7 +240,7 @@ PALZ_HD void solve_qr(F* A, F* b, F* x) {
 // conversion is defined for 0 <= c * 2^32 < 2^64 only, so a NaN or
 // non-positive c gives 0 and c >= 2^32 saturates at the largest float below
 // 2^64 (with the radiance clamp checked at the API neither occurs).
-PALZ_HD inline unsigned long long fixed_word(float c) {
+PALZ_HD unsigned long long fixed_word(float c) {
     if (!(c > 0.0f)) return 0ull;
     const float s = c < 4294967296.0f ? c * 4294967296.0f : 18446742974197923840.0f;
     return (unsigned long long)s;

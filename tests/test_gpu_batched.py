@@ -377,8 +377,31 @@ def test_interval_cull_matches_oracle(run, far):
     print(f"{run.name} far={far}: culled {np.sum(got == 0)} of {len(got)}")
 
 
-def test_batched_culling_keeps_every_contribution(run):
-    run.bs.set_culling(True)
+@pytest.mark.parametrize("far", [False, True])
+def test_affine_cull_keeps_a_subset_of_the_interval_keeps(run, far):
+    """The affine test runs on what the interval test keeps, so it keeps a
+    subset of it; no tuple it cuts holds a path the solver makes contribute."""
+    paths = run.paths.copy()
+    if far:
+        n_pts = run.s.points.shape[0]
+        paths["pixel"] = (paths["pixel"] + n_pts // 2 + run.s.width // 2) % n_pts
+    d_paths = DeviceBuffer.from_array(paths)
+    iv_keep = CA.interval_cull(run.d_scenes, 1, d_paths, paths.shape[0])
+    af_keep = CA.affine_cull(run.d_scenes, 1, d_paths, paths.shape[0])
+    assert np.all(af_keep <= iv_keep)
+    K = paths["event"] >> 24
+    assert np.all(af_keep[K > 3] == 1)
+    if not far:
+        keep, want, _ = run.oracle()
+        contributes = (want["status"] == CA.PATH_CONTRIBUTES).reshape(-1, run.L).any(axis=1)
+        assert not np.any(contributes & (af_keep[keep] == 0)), "a contributing path was culled"
+    print(f"{run.name} far={far}: interval keeps {int(iv_keep.sum())}, affine keeps {int(af_keep.sum())} "
+          f"of {len(af_keep)}")
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batched_culling_keeps_every_contribution(run, mode):
+    run.bs.set_culling(mode)
     try:
         res, _, img = run.solve(refine=CA.REFINE_NONE)
         st = run.bs.stats()
