@@ -209,7 +209,7 @@ def bench_c2(args) -> dict:
     else:
         log("[bench] c2: autotune")
         tune = bs.autotune(d_scenes, 1, L, refine=CA.REFINE_VERTICES)
-        log(f"[bench] c2: tuned {tune}")
+        log(f"[bench] c2: tuned {tune}, fused {bs.get_fused()[1]}")
 
     # batched, timed region: eager launches with per-kernel events on the stream
     for _ in range(args.warmup):
@@ -298,6 +298,8 @@ def bench_c2(args) -> dict:
     t1.record(st)
     st.sync()
     ms_graph = t0.elapsed_ms(t1) / args.steps
+    log("[bench] c2: fused mode")
+    fused = bench_fused(args, bs, d_scenes, 1, L, d_res, st, graph=False, check=False)
     g.clear_image()
     bs.solve(d_scenes, 1, L, CA.REFINE_VERTICES, d_res)
     img_batched = g.image().reshape(-1).astype(np.float64)
@@ -431,7 +433,8 @@ def bench_c2(args) -> dict:
                     "kernel_ms": {k: round(v, 3) for k, v in kern.items()},
                     "converged_items": converged, "contributing_items": contributing,
                     "stats": stats, "launch_configs": tune, "wall_s": round(wall, 3),
-                    "newton_fp32_roofline": valu},
+                    "newton_fp32_roofline": valu, "fused": fused,
+                    "fused_chosen_by_autotuner": bs.get_fused()[1]},
         "plain": {"value": round(plain_value, 1), "unit": "paths/s", "ms_per_step": round(ms_plain, 3),
                   "launches": len(plain_lists), "kernel_ms": {k: round(v, 3) for k, v in plain_kernel_ms.items()},
                   "batched_over_plain": round(value / plain_value, 2) if plain_value else None,
@@ -575,6 +578,7 @@ def bench_c4(args) -> dict:
     ranges = bs.ranges()
     assert ranges[3] == sum(x[4] for x in plain)
     tune = bs.autotune(batch.d_scenes, len(scene_list), L, refine=CA.REFINE_VERTICES)
+    fused_chosen = bs.get_fused()[1]
     d_res = DeviceBuffer(max(n * L, 1) * CA.PATH_RESULT.itemsize)
     t0, t1 = Event(), Event()
     for _ in range(args.warmup + 1):
@@ -588,6 +592,7 @@ def bench_c4(args) -> dict:
     ms = t0.elapsed_ms(t1) / args.steps
     stats = bs.stats()
     contributing = sum(stats[f"K{k}"]["contributing"] for k in (1, 2, 3))
+    fused = bench_fused(args, bs, batch.d_scenes, len(scene_list), L, d_res, st, graph=True, check=True)
     for _ in range(max(1, args.warmup)):
         for gs, K, ev, d, m in plain:
             gs.solve_plain(K, ev, d, m)
@@ -619,7 +624,47 @@ def bench_c4(args) -> dict:
             "batched_graph_ms": round(ms, 3), "plain_ms": round(ms_plain, 3), "plain_launches": len(plain),
             "batched_paths_per_s": round(contributing / (ms * 1e-3), 1),
             "plain_paths_per_s": round(contributing / (ms_plain * 1e-3), 1),
-            "batched_over_plain": round(ms_plain / ms, 2), "launch_configs": tune}
+            "batched_over_plain": round(ms_plain / ms, 2), "launch_configs": tune,
+            "fused_chosen_by_autotuner": fused_chosen, "fused": fused}
+
+
+def bench_fused(args, bs, d_scenes, n_scenes, L, d_res, st, graph: bool, check: bool) -> dict:
+    """The same solve in fused mode (one persistent Newton kernel and one
+    occlusion kernel over every chain length, shapes from the autotuner):
+    time per step, per-kernel times of eager solves and, with `check`, whether
+    every per-item result equals the per-length kernels' bytes (d_res holds
+    those; without `check` it is overwritten)."""
+    from palzilla_amd import caustics as CA
+    from palzilla_amd.device import DeviceBuffer, Event
+    ref = CA.download_results(d_res, int(bs.ranges()[5]) * L) if check else None
+    bs.set_fused(True)
+    try:
+        bs.autotune(d_scenes, n_scenes, L, refine=CA.REFINE_VERTICES)
+        shape = bs.get_config(0)
+        d_out = DeviceBuffer(d_res.nbytes) if check else d_res
+        for _ in range(args.warmup + 1):
+            bs.solve(d_scenes, n_scenes, L, CA.REFINE_VERTICES, d_out, graph=graph)
+        t0, t1 = Event(), Event()
+        st.sync()
+        t0.record(st)
+        for _ in range(args.steps):
+            bs.solve(d_scenes, n_scenes, L, CA.REFINE_VERTICES, d_out, graph=graph)
+        t1.record(st)
+        st.sync()
+        ms = t0.elapsed_ms(t1) / args.steps
+        bs.set_timing(True)
+        bs.solve(d_scenes, n_scenes, L, CA.REFINE_VERTICES, d_out)
+        k = bs.timings()
+        bs.set_timing(False)
+        same = None
+        if ref is not None:
+            same = CA.download_results(d_out, ref.shape[0]).tobytes() == ref.tobytes()
+    finally:
+        bs.set_fused(-1)
+    return {"ms_per_step": round(ms, 3), "graph": graph, "results_equal_per_length_kernels": same,
+            "newton_mixed_ms": round(k["newton_k1"], 3), "occlude_mixed_ms": round(k["occlude_k1"], 3),
+            "finish_ms": {f"K{K}": round(k[f"finish_k{K}"], 3) for K in (1, 2, 3)},
+            "compact_ms": round(k["compact"], 3), "shape": shape}
 
 
 # ---------------------------------------------------------------------------------------------------

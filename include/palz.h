@@ -611,8 +611,22 @@ int palz_batched_prepare(palz_batched* b, const palz_path* d_paths, uint32_t n_p
                          palz_sorter* sorter, void* stream);
 int palz_batched_order(palz_batched* b, const uint32_t** d_order, const palz_path** d_sorted_paths,
                        const uint32_t** d_ranges);
+/* K = 1..3: the kernels of chain length K.  K = 0: the mixed-length kernels
+ * of fused mode: block and blocks_per_cu of the Newton kernel, finish_block
+ * of the occlusion kernel, and in `chunk` the Newton kernel's register budget
+ * in waves per SIMD, 2 or 3 (each length keeps its own queue chunk). */
 int palz_batched_set_config(palz_batched* b, int K, const palz_launch_config* cfg);
 int palz_batched_get_config(palz_batched* b, int K, palz_launch_config* cfg);
+/* Fused mode (no counterpart in the reference): 1 = one persistent Newton
+ * kernel and one occlusion kernel over every chain length (each wave drains
+ * the K = 3 queue, then K = 2's, then K = 1's) in place of one kernel per
+ * length; the finish kernels stay per length.  0 = one Newton and one
+ * occlusion kernel per chain length.  -1 (default) = per length until
+ * palz_batched_autotune has timed whole solves both ways and kept the faster.
+ * Results are the same either way.  Drops the captured graph. */
+int palz_batched_set_fused(palz_batched* b, int mode);
+/* The fused mode set and whether solves now run fused (0 or 1). */
+int palz_batched_get_fused(palz_batched* b, int* mode, int* active);
 /* Solve every prepared item.  d_scenes: device array of n_scenes scene
  * descriptors (a path's `scene` indexes it); every scene's lambdas array
  * holds at least n_lambdas entries.  d_results (optional) is indexed by batch
@@ -677,6 +691,14 @@ int palz_batched_set_culling(palz_batched* b, int mode);
  * newton_ms * 1000, finish_ms * 1000}) if not NULL. */
 int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint32_t n_scenes, uint32_t n_lambdas,
                           int refine, int32_t* report, void* stream);
+/* The autotuner's tile-shape cache (no counterpart in the reference): with a
+ * file set, palz_batched_autotune first looks the workload up there (key: CU
+ * count, scene count and per-length item counts rounded up to powers of two,
+ * n_lambdas, refine, culling and fused modes) and applies the stored shapes
+ * (and, in fused mode -1, the stored choice) without timing anything;
+ * otherwise it tunes and appends one line.  NULL turns the cache off (the
+ * default). */
+int palz_batched_set_tune_cache(palz_batched* b, const char* path);
 
 #ifdef __cplusplus
 }

@@ -117,6 +117,9 @@ _SIGS = {
     "palz_batched_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "palz_batched_timings": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "palz_batched_set_culling": (C.c_int, [C.c_void_p, C.c_int]),
+    "palz_batched_set_fused": (C.c_int, [C.c_void_p, C.c_int]),
+    "palz_batched_get_fused": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "palz_batched_set_tune_cache": (C.c_int, [C.c_void_p, C.c_char_p]),
     "palz_batched_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int,
                                         C.POINTER(C.c_int32), C.c_void_p]),
     "palz_lbvh_set_timing": (C.c_int, [C.c_void_p, C.c_int]),

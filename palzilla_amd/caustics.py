@@ -380,6 +380,23 @@ class BatchedSolver:
         out["bad_scene"] = int(a[14])
         return out
 
+    def set_fused(self, mode: int | bool) -> None:
+        """1/True: one persistent Newton and one occlusion kernel over every
+        chain length; 0/False: one of each per chain length; -1 (the default):
+        chosen by the autotuner (per chain length until it has run)."""
+        call("palz_batched_set_fused", C.c_void_p(self.handle), int(mode))
+
+    def get_fused(self) -> tuple[int, bool]:
+        """(mode set, whether solves now run fused)."""
+        mode, active = C.c_int(), C.c_int()
+        call("palz_batched_get_fused", C.c_void_p(self.handle), C.byref(mode), C.byref(active))
+        return mode.value, bool(active.value)
+
+    def set_tune_cache(self, path: str | None) -> None:
+        """The autotuner's tile-shape cache file (None: no cache)."""
+        call("palz_batched_set_tune_cache", C.c_void_p(self.handle),
+             None if path is None else os.fsencode(path))
+
     def set_branches(self, mode: int) -> None:
         """Captured solves as three graph branches (1), one stream (0) or
         decided from the chain-length mix (-1, the default)."""

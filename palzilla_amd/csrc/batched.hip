@@ -22,7 +22,11 @@
 //     solve can be captured in a hipGraph (counters are reset by a kernel, no
 //     memset nodes).
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 
 #include "cull.hpp"
 #include "optics.hpp"
@@ -128,15 +132,16 @@ __device__ __forceinline__ void wave_add_stat(unsigned long long* dst, unsigned 
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
 }
 
-// Persistent Newton iterations of every item of chain length K.  For K = 2
-// the register budget is held at 168 so that three waves share a SIMD
-// (K = 3 would spill).
+// Persistent Newton iterations of every item of chain length K, run by the
+// calling wave until chain length K's queue is drained.
 template <int K>
-__global__ void __launch_bounds__(256, K == 2 ? 3 : 1)
-    k_newton(const palz_scene_desc* __restrict__ scenes, uint32_t n_scenes, const palz_path* __restrict__ paths,
-             const uint32_t* __restrict__ ranges, uint32_t L, uint32_t chunk, Counters* __restrict__ ctr,
-             uint32_t* __restrict__ flags, float* __restrict__ conv_params, int* __restrict__ conv_iters,
-             const uint8_t* __restrict__ admissible, palz_path_result* __restrict__ results) {
+__device__ __forceinline__ void newton_persistent(const palz_scene_desc* __restrict__ scenes, uint32_t n_scenes,
+                                                  const palz_path* __restrict__ paths,
+                                                  const uint32_t* __restrict__ ranges, uint32_t L, uint32_t chunk,
+                                                  Counters* __restrict__ ctr, uint32_t* __restrict__ flags,
+                                                  float* __restrict__ conv_params, int* __restrict__ conv_iters,
+                                                  const uint8_t* __restrict__ admissible,
+                                                  palz_path_result* __restrict__ results) {
     constexpr int P = 2 * K;
     constexpr int R = 3 * K;
     const uint32_t first_path = ranges[K - 1];
@@ -268,6 +273,40 @@ __global__ void __launch_bounds__(256, K == 2 ? 3 : 1)
     wave_add_stat(&ctr->stats[12], n_culled);
 }
 
+// The Newton kernel of chain length K.  For K = 2 the register budget is held
+// at 168 so that three waves share a SIMD (K = 3 would spill).
+template <int K>
+__global__ void __launch_bounds__(256, K == 2 ? 3 : 1)
+    k_newton(const palz_scene_desc* __restrict__ scenes, uint32_t n_scenes, const palz_path* __restrict__ paths,
+             const uint32_t* __restrict__ ranges, uint32_t L, uint32_t chunk, Counters* __restrict__ ctr,
+             uint32_t* __restrict__ flags, float* __restrict__ conv_params, int* __restrict__ conv_iters,
+             const uint8_t* __restrict__ admissible, palz_path_result* __restrict__ results) {
+    newton_persistent<K>(scenes, n_scenes, paths, ranges, L, chunk, ctr, flags, conv_params, conv_iters, admissible,
+                         results);
+}
+
+// One persistent kernel over every chain length: each wave drains the K = 3
+// queue, then K = 2's, then K = 1's (the longest chains first, so that the
+// tail is made of the cheapest items).  A wave that finds one queue empty
+// moves on while others still work on it, so no chain length waits for
+// another's last wave.  One register allocation serves every body: budgeted
+// for kWaves waves per SIMD, it is K = 3's 244 VGPRs at 2 waves (K = 2 then
+// runs at 2 waves too) or 168 at 3 waves (K = 2's own budget; K = 3 spills).
+template <int kWaves>
+__global__ void __launch_bounds__(256, kWaves)
+    k_newton_mixed(const palz_scene_desc* __restrict__ scenes, uint32_t n_scenes,
+                   const palz_path* __restrict__ paths, const uint32_t* __restrict__ ranges, uint32_t L,
+                   uint32_t chunk1, uint32_t chunk2, uint32_t chunk3, Counters* __restrict__ ctr,
+                   uint32_t* __restrict__ flags, float* __restrict__ conv_params, int* __restrict__ conv_iters,
+                   const uint8_t* __restrict__ admissible, palz_path_result* __restrict__ results) {
+    newton_persistent<3>(scenes, n_scenes, paths, ranges, L, chunk3, ctr, flags, conv_params, conv_iters, admissible,
+                         results);
+    newton_persistent<2>(scenes, n_scenes, paths, ranges, L, chunk2, ctr, flags, conv_params, conv_iters, admissible,
+                         results);
+    newton_persistent<1>(scenes, n_scenes, paths, ranges, L, chunk1, ctr, flags, conv_params, conv_iters, admissible,
+                         results);
+}
+
 // Throughput, FP64 refinement, dA/dw and irradiance of the converged chains.
 __device__ __forceinline__ uint32_t compact_pos(const uint32_t* excl, uint32_t j, uint32_t n, uint32_t total) {
     return j < n ? excl[j] : total;
@@ -321,24 +360,28 @@ constexpr uint32_t kOccludeGrabMax = 256;
 // Most distinct paths in a wave for which the occlusion rays walk as one packet.
 constexpr uint32_t kPacketPaths = 16;
 
-// Clears chain length K's per-XCD counters before its k_occlude (a kernel, not
-// a memset, so that it can be captured in a graph).
+// Clears chain length K's per-XCD counters before its k_occlude, or with
+// K = 0 every length's (a kernel, not a memset, so that it can be captured in
+// a graph).
 __global__ void k_clear_occlude(Counters* c, int K) {
-    if (threadIdx.x < kXcds) c->occlude[K - 1][threadIdx.x] = 0u;
+    if (threadIdx.x < kXcds * 3u && (K == 0 || (int)(threadIdx.x / kXcds) == K - 1))
+        c->occlude[threadIdx.x / kXcds][threadIdx.x % kXcds] = 0u;
 }
 
+// The occlusion rays of chain length K's converged items, run by the calling
+// wave until that length's list is drained.  `pstack`: the wave's LDS stack.
 template <int K>
-__global__ void __launch_bounds__(256, 8)  // 8 waves per SIMD: latency-bound, few live registers
-    k_occlude(const palz_scene_desc* __restrict__ scenes, const palz_path* __restrict__ paths,
-              const uint32_t* __restrict__ ranges, uint32_t L, Counters* __restrict__ ctr,
-              const uint32_t* __restrict__ excl, const uint32_t* __restrict__ list,
-              const float* __restrict__ conv_params, float* __restrict__ thr_out) {
+__device__ __forceinline__ void occlude_persistent(const palz_scene_desc* __restrict__ scenes,
+                                                   const palz_path* __restrict__ paths,
+                                                   const uint32_t* __restrict__ ranges, uint32_t L,
+                                                   Counters* __restrict__ ctr, const uint32_t* __restrict__ excl,
+                                                   const uint32_t* __restrict__ list,
+                                                   const float* __restrict__ conv_params,
+                                                   float* __restrict__ thr_out, PacketStack* pstack) {
     constexpr int P = 2 * K;
     const uint32_t n_items = ctr->n_items, total = ctr->converged;
     const uint32_t begin = compact_pos(excl, ranges[K - 1] * L, n_items, total);
     const uint32_t end = compact_pos(excl, ranges[K] * L, n_items, total);
-    __shared__ PacketStack packet_stacks[256 / 64];
-    PacketStack* pstack = &packet_stacks[threadIdx.x >> 6];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t waves = gridDim.x * (blockDim.x / 64u);
     const uint32_t grab_size = 64u * min(kOccludeGrabMax / 64u, max(1u, (end - begin) / (waves * 64u * 8u)));
@@ -405,6 +448,31 @@ __global__ void __launch_bounds__(256, 8)  // 8 waves per SIMD: latency-bound, f
             }
         }
     }
+}
+
+template <int K>
+__global__ void __launch_bounds__(256, 8)  // 8 waves per SIMD: latency-bound, few live registers
+    k_occlude(const palz_scene_desc* __restrict__ scenes, const palz_path* __restrict__ paths,
+              const uint32_t* __restrict__ ranges, uint32_t L, Counters* __restrict__ ctr,
+              const uint32_t* __restrict__ excl, const uint32_t* __restrict__ list,
+              const float* __restrict__ conv_params, float* __restrict__ thr_out) {
+    __shared__ PacketStack packet_stacks[256 / 64];
+    occlude_persistent<K>(scenes, paths, ranges, L, ctr, excl, list, conv_params, thr_out,
+                          &packet_stacks[threadIdx.x >> 6]);
+}
+
+// The occlusion rays of every chain length in one persistent kernel: each
+// wave drains the K = 3 list, then K = 2's, then K = 1's.
+__global__ void __launch_bounds__(256, 8)
+    k_occlude_mixed(const palz_scene_desc* __restrict__ scenes, const palz_path* __restrict__ paths,
+                    const uint32_t* __restrict__ ranges, uint32_t L, Counters* __restrict__ ctr,
+                    const uint32_t* __restrict__ excl, const uint32_t* __restrict__ list,
+                    const float* __restrict__ conv_params, float* __restrict__ thr_out) {
+    __shared__ PacketStack packet_stacks[256 / 64];
+    PacketStack* ps = &packet_stacks[threadIdx.x >> 6];
+    occlude_persistent<3>(scenes, paths, ranges, L, ctr, excl, list, conv_params, thr_out, ps);
+    occlude_persistent<2>(scenes, paths, ranges, L, ctr, excl, list, conv_params, thr_out, ps);
+    occlude_persistent<1>(scenes, paths, ranges, L, ctr, excl, list, conv_params, thr_out, ps);
 }
 
 // dA/dw and irradiance of the converged chains of length K, in item order,
@@ -542,6 +610,21 @@ void launch_occlude(const palz_launch_config& c, int cus, hipStream_t st, const 
     k_clear_occlude<<<1, 64, 0, st>>>(ctr, K);
     k_occlude<K><<<cus * 8, c.finish_block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, thr);
 }
+void launch_newton_mixed(const palz_launch_config& m, const palz_launch_config* cfg, int cus, hipStream_t st,
+                         const palz_scene_desc* scenes, uint32_t n_scenes, const palz_path* paths,
+                         const uint32_t* ranges, uint32_t L, Counters* ctr, uint32_t* flags, float* cp, int* cit,
+                         const uint8_t* admissible, palz_path_result* results) {
+    auto kernel = m.chunk == 3 ? k_newton_mixed<3> : k_newton_mixed<2>;
+    kernel<<<m.blocks_per_cu * cus, m.block, 0, st>>>(scenes, n_scenes, paths, ranges, L, (uint32_t)cfg[0].chunk,
+                                                      (uint32_t)cfg[1].chunk, (uint32_t)cfg[2].chunk, ctr, flags, cp,
+                                                      cit, admissible, results);
+}
+void launch_occlude_mixed(const palz_launch_config& m, int cus, hipStream_t st, const palz_scene_desc* scenes,
+                          const palz_path* paths, const uint32_t* ranges, uint32_t L, Counters* ctr,
+                          const uint32_t* excl, const uint32_t* list, const float* cp, float* thr) {
+    k_clear_occlude<<<1, 64, 0, st>>>(ctr, 0);
+    k_occlude_mixed<<<cus * 8, m.finish_block, 0, st>>>(scenes, paths, ranges, L, ctr, excl, list, cp, thr);
+}
 template <int K>
 void launch_finish(const palz_launch_config& c, int cus, hipStream_t st, const palz_scene_desc* scenes,
                    const palz_path* paths, const uint32_t* ranges, uint32_t L, Counters* ctr, const uint32_t* excl,
@@ -581,6 +664,17 @@ struct palz_batched {
     uint32_t* scan_sums = nullptr;
     size_t cap_items = 0;
     palz_launch_config cfg[3] = {{256, 4, 256, 128}, {256, 2, 256, 128}, {128, 2, 256, 128}};
+    // the mixed-length kernels' shape (block, blocks_per_cu, the Newton
+    // kernel's register budget in waves per SIMD (2 or 3), occlusion block);
+    // each length keeps its own queue chunk from cfg
+    palz_launch_config cfg_mixed = {256, 2, 2, 256};
+    // fused mode: 1 = one persistent Newton and one occlusion kernel over every
+    // chain length, 0 = one of each per length, -1 = chosen by the autotuner
+    // (per length until it has run)
+    int fused = -1;
+    bool auto_fused = false;  // the autotuner's choice in mode -1
+    bool fused_on() const { return fused == 1 || (fused == -1 && auto_fused); }
+    std::string tune_cache;  // the autotuner's tile-shape cache file ("" = none)
     int cus = 256;
     // hipGraph cache
     hipGraph_t graph = nullptr;
@@ -603,6 +697,8 @@ struct palz_batched {
         int refine;
         palz_launch_config cfg[3];
         int cull;
+        int fused;
+        palz_launch_config cfg_mixed;
     } key{};
 
     int ensure_paths(uint32_t n) {
@@ -677,6 +773,14 @@ struct palz_batched {
         palz::launch_finish<K>(c, cus, st, scenes, paths, ranges, L, ctr, flags, list, conv_params, conv_iters, thr,
                                refine, results, refined);
     }
+    void newton_mixed(const palz_launch_config& m, const palz_scene_desc* scenes, uint32_t n_scenes, uint32_t L,
+                      palz_path_result* results, hipStream_t st) {
+        palz::launch_newton_mixed(m, cfg, cus, st, scenes, n_scenes, paths, ranges, L, ctr, flags, conv_params,
+                                  conv_iters, cull ? admissible : nullptr, results);
+    }
+    void occlude_mixed(const palz_launch_config& m, const palz_scene_desc* scenes, uint32_t L, hipStream_t st) {
+        palz::launch_occlude_mixed(m, cus, st, scenes, paths, ranges, L, ctr, flags, list, conv_params, thr);
+    }
     // K = 4 marks, the scan of the converged flags and the scatter into the list
     void compact(uint32_t L, palz_path_result* results, hipStream_t st) {
         using namespace palz;
@@ -732,6 +836,28 @@ struct palz_batched {
         if (t) PALZ_HIP(hipEventRecord(ev[0], st));
         if (cull) launch_cull_paths(scenes, n_scenes, paths, cap_paths, ranges + 5, admissible, st, cull);
         if (t) PALZ_HIP(hipEventRecord(ev[9], st));
+        if (fused_on()) {
+            // the fused kernels' times go to the K = 1 slots; K = 2 and 3 stay ~0
+            newton_mixed(cfg_mixed, scenes, n_scenes, L, results, st);
+            for (int i : {1, 2, 3})
+                if (t) PALZ_HIP(hipEventRecord(ev[i], st));
+            compact(L, results, st);
+            if (t) PALZ_HIP(hipEventRecord(ev[4], st));
+            occlude_mixed(cfg_mixed, scenes, L, st);
+            if (t) PALZ_HIP(hipEventRecord(ev[10], st));
+            finish<1>(cfg[0], scenes, L, refine, results, refined, st);
+            if (t) PALZ_HIP(hipEventRecord(ev[5], st));
+            if (t) PALZ_HIP(hipEventRecord(ev[11], st));
+            finish<2>(cfg[1], scenes, L, refine, results, refined, st);
+            if (t) PALZ_HIP(hipEventRecord(ev[6], st));
+            if (t) PALZ_HIP(hipEventRecord(ev[12], st));
+            finish<3>(cfg[2], scenes, L, refine, results, refined, st);
+            if (t) PALZ_HIP(hipEventRecord(ev[7], st));
+            if (t) PALZ_HIP(hipEventRecord(ev[8], st));
+            PALZ_LAUNCH_CHECK();
+            if (t) timed_solves = true;
+            return PALZ_OK;
+        }
         newton<1>(cfg[0], scenes, n_scenes, L, results, st);
         if (t) PALZ_HIP(hipEventRecord(ev[1], st));
         newton<2>(cfg[1], scenes, n_scenes, L, results, st);
@@ -760,6 +886,86 @@ struct palz_batched {
 };
 
 namespace {
+bool valid_config(const palz_launch_config& c, bool mixed = false) {
+    auto ok_block = [](int v) { return v == 64 || v == 128 || v == 256; };
+    const bool ok_chunk = mixed ? (c.chunk == 2 || c.chunk == 3) : (c.chunk >= 1 && c.chunk <= (1 << 16));
+    return ok_block(c.block) && ok_block(c.finish_block) && c.blocks_per_cu >= 1 && c.blocks_per_cu <= 32 && ok_chunk;
+}
+
+// The autotuner's tile-shape cache.  A workload is keyed by the device's CU
+// count, the batch's scene count (rounded up to a power of two), wavelength
+// count, refinement, culling and fused modes and, per chain length, its item
+// count rounded up to a power of two; workloads with equal keys take the same
+// launch shapes.  The file holds one line per tuning,
+//     palz-tune <10 key fields> : <cfg K=1,2,3 and mixed: 4 x 4 fields> <6 times in us> <fused chosen>
+// and the last line with a matching key wins.
+constexpr int kTuneKeyFields = 10;
+constexpr int kTuneFields = 4 * 4 + 6 + 1;
+struct TuneKey {
+    int v[kTuneKeyFields];
+};
+int ceil_log2_plus1(uint64_t x) {
+    int r = 0;
+    while (r < 63 && (1ull << r) < x + 1ull) ++r;
+    return r;
+}
+TuneKey tune_key(int cus, const uint32_t* ranges, uint32_t n_scenes, uint32_t L, int refine, int cull, int fused) {
+    TuneKey k;
+    k.v[0] = 2;  // format version
+    k.v[1] = cus;
+    k.v[2] = ceil_log2_plus1(n_scenes);
+    k.v[3] = (int)L;
+    k.v[4] = refine;
+    k.v[5] = cull;
+    k.v[6] = fused;
+    for (int K = 1; K <= 3; ++K) k.v[6 + K] = ceil_log2_plus1((uint64_t)(ranges[K] - ranges[K - 1]) * L);
+    return k;
+}
+bool parse_ints(const char** p, int n, long* out) {
+    for (int i = 0; i < n; ++i) {
+        char* e = nullptr;
+        const long x = strtol(*p, &e, 10);
+        if (e == *p) return false;
+        out[i] = x;
+        *p = e;
+    }
+    return true;
+}
+bool tune_cache_lookup(const std::string& path, const TuneKey& key, int32_t* vals) {
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return false;
+    char line[1024];
+    bool found = false;
+    while (fgets(line, sizeof line, f)) {
+        const char* p = line;
+        if (strncmp(p, "palz-tune ", 10) != 0) continue;
+        p += 10;
+        long kv[kTuneKeyFields], v[kTuneFields];
+        if (!parse_ints(&p, kTuneKeyFields, kv)) continue;
+        while (*p == ' ') ++p;
+        if (*p != ':') continue;
+        ++p;
+        if (!parse_ints(&p, kTuneFields, v)) continue;
+        bool same = true;
+        for (int i = 0; i < kTuneKeyFields; ++i) same = same && kv[i] == key.v[i];
+        if (!same) continue;
+        for (int i = 0; i < kTuneFields; ++i) vals[i] = (int32_t)v[i];
+        found = true;
+    }
+    fclose(f);
+    return found;
+}
+bool tune_cache_append(const std::string& path, const TuneKey& key, const int32_t* vals) {
+    FILE* f = fopen(path.c_str(), "a");
+    if (!f) return false;
+    fprintf(f, "palz-tune");
+    for (int i = 0; i < kTuneKeyFields; ++i) fprintf(f, " %d", key.v[i]);
+    fprintf(f, " :");
+    for (int i = 0; i < kTuneFields; ++i) fprintf(f, " %d", (int)vals[i]);
+    fprintf(f, "\n");
+    return fclose(f) == 0;
+}
+
 int check_solve_args(palz_batched* b, const palz_scene_desc* d_scenes, uint32_t n_scenes, uint32_t L,
                      int refine) {
     using namespace palz;
@@ -842,21 +1048,34 @@ int palz_batched_order(palz_batched* b, const uint32_t** d_order, const palz_pat
 
 int palz_batched_set_config(palz_batched* b, int K, const palz_launch_config* cfg) {
     using namespace palz;
-    if (!b || !cfg || K < 1 || K > 3) return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_set_config: bad argument");
+    if (!b || !cfg || K < 0 || K > 3) return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_set_config: bad argument");
     const palz_launch_config& c = *cfg;
-    auto ok_block = [](int v) { return v == 64 || v == 128 || v == 256; };
-    if (!ok_block(c.block) || !ok_block(c.finish_block) || c.blocks_per_cu < 1 || c.blocks_per_cu > 32 ||
-        c.chunk < 1 || c.chunk > (1 << 16))
+    if (!valid_config(c, K == 0))
         return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_set_config: block/finish_block in {64,128,256}, "
-                                               "blocks_per_cu in 1..32, chunk in 1..65536");
-    b->cfg[K - 1] = c;
+                                               "blocks_per_cu in 1..32, chunk in 1..65536 (K = 0: 2 or 3)");
+    (K == 0 ? b->cfg_mixed : b->cfg[K - 1]) = c;
     b->drop_graph();
     return PALZ_OK;
 }
 
 int palz_batched_get_config(palz_batched* b, int K, palz_launch_config* cfg) {
-    if (!b || !cfg || K < 1 || K > 3) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_get_config: bad argument");
-    *cfg = b->cfg[K - 1];
+    if (!b || !cfg || K < 0 || K > 3) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_get_config: bad argument");
+    *cfg = K == 0 ? b->cfg_mixed : b->cfg[K - 1];
+    return PALZ_OK;
+}
+
+int palz_batched_set_fused(palz_batched* b, int mode) {
+    if (!b || mode < -1 || mode > 1)
+        return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_set_fused: mode must be -1, 0 or 1");
+    b->fused = mode;
+    b->drop_graph();
+    return PALZ_OK;
+}
+
+int palz_batched_get_fused(palz_batched* b, int* mode, int* active) {
+    if (!b) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_get_fused: NULL batch");
+    if (mode) *mode = b->fused;
+    if (active) *active = b->fused_on() ? 1 : 0;
     return PALZ_OK;
 }
 
@@ -873,7 +1092,8 @@ int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, u
     int rc = check_solve_args(b, d_scenes, n_scenes, n_lambdas, refine);
     if (rc != PALZ_OK) return rc;
     hipStream_t st = (hipStream_t)stream;
-    palz_batched::Key k{d_scenes, d_results, d_refined, n_scenes, n_lambdas, refine, {}, b->cull};
+    palz_batched::Key k{d_scenes, d_results, d_refined, n_scenes, n_lambdas, refine, {}, b->cull, b->fused_on(),
+                        b->cfg_mixed};
     std::copy(b->cfg, b->cfg + 3, k.cfg);
     const bool same = b->exec && std::memcmp(&k, &b->key, sizeof(k)) == 0;
     if (!same) {
@@ -881,8 +1101,8 @@ int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, u
         // Branches pay when no chain length dominates (many scenes of mixed
         // lengths); when one length holds most paths its kernels fill the GPU
         // and the others only compete with them.
-        bool branch = b->branches == 1;
-        if (b->branches < 0) {
+        bool branch = b->branches == 1 && !b->fused_on();
+        if (b->branches < 0 && !b->fused_on()) {
             uint32_t r[6];
             PALZ_HIP(hipMemcpy(r, b->ranges, sizeof(r), hipMemcpyDeviceToHost));
             const uint32_t most = std::max(r[1] - r[0], std::max(r[2] - r[1], r[3] - r[2]));
@@ -964,6 +1184,12 @@ int palz_batched_path_stats(palz_batched* b, uint64_t* converged_paths3, void* s
     return PALZ_OK;
 }
 
+int palz_batched_set_tune_cache(palz_batched* b, const char* path) {
+    if (!b) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_set_tune_cache: NULL batch");
+    b->tune_cache = path ? path : "";
+    return PALZ_OK;
+}
+
 int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint32_t n_scenes, uint32_t n_lambdas,
                           int refine, int32_t* report, void* stream) {
     using namespace palz;
@@ -973,6 +1199,42 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
     uint32_t ranges[6];
     PALZ_HIP(hipStreamSynchronize(st));
     PALZ_HIP(hipMemcpy(ranges, b->ranges, sizeof(ranges), hipMemcpyDeviceToHost));
+    const TuneKey key = tune_key(b->cus, ranges, n_scenes, n_lambdas, refine, b->cull, b->fused);
+    float newton_ms[3] = {0.0f, 0.0f, 0.0f}, finish_ms[3] = {0.0f, 0.0f, 0.0f};
+    auto write_report = [&]() {
+        if (!report) return;
+        for (int K = 1; K <= 3; ++K) {
+            const palz_launch_config& c = b->cfg[K - 1];
+            int32_t* r = report + (K - 1) * 6;
+            r[0] = c.block;
+            r[1] = c.blocks_per_cu;
+            r[2] = c.chunk;
+            r[3] = c.finish_block;
+            r[4] = (int32_t)(newton_ms[K - 1] * 1000.0f);
+            r[5] = (int32_t)(finish_ms[K - 1] * 1000.0f);
+        }
+    };
+    if (!b->tune_cache.empty()) {
+        int32_t v[kTuneFields];
+        if (tune_cache_lookup(b->tune_cache, key, v)) {
+            palz_launch_config c[4];
+            for (int i = 0; i < 4; ++i) c[i] = {v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]};
+            const bool ok = valid_config(c[0]) && valid_config(c[1]) && valid_config(c[2]) &&
+                            valid_config(c[3], true) && (v[22] == 0 || v[22] == 1);
+            if (ok) {
+                std::copy(c, c + 3, b->cfg);
+                b->cfg_mixed = c[3];
+                b->auto_fused = v[22] == 1;
+                for (int K = 0; K < 3; ++K) {
+                    newton_ms[K] = (float)v[16 + 2 * K] / 1000.0f;
+                    finish_ms[K] = (float)v[17 + 2 * K] / 1000.0f;
+                }
+                write_report();
+                b->drop_graph();
+                return PALZ_OK;
+            }
+        }
+    }
     hipEvent_t e0, e1;
     PALZ_HIP(hipEventCreate(&e0));
     PALZ_HIP(hipEventCreate(&e1));
@@ -991,7 +1253,6 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
         if (K == 2) b->newton<2>(c, d_scenes, n_scenes, n_lambdas, nullptr, st);
         if (K == 3) b->newton<3>(c, d_scenes, n_scenes, n_lambdas, nullptr, st);
     };
-    float newton_ms[3] = {0.0f, 0.0f, 0.0f}, finish_ms[3] = {0.0f, 0.0f, 0.0f};
     if (b->cull)
         launch_cull_paths(d_scenes, n_scenes, b->paths, b->cap_paths, b->ranges + 5, b->admissible, st, b->cull);
     // Newton kernels: block size, resident blocks per CU and queue chunk
@@ -1021,6 +1282,30 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
         b->cfg[K - 1] = best;
         newton_ms[K - 1] = best_ms;
     }
+    // the mixed-length Newton kernel: register budget (waves per SIMD), block
+    // size and resident blocks per CU, with each length's tuned queue chunk
+    if (b->fused != 0) {
+        float best_ms = 1e30f;
+        palz_launch_config best = b->cfg_mixed;
+        for (int waves : {2, 3})
+            for (int blk : blocks)
+                for (int pc : per_cu) {
+                    if (blk * pc < 256 || blk * pc > 2048) continue;
+                    palz_launch_config c = best;
+                    c.block = blk;
+                    c.blocks_per_cu = pc;
+                    c.chunk = waves;
+                    k_reset<<<1, 64, 0, st>>>(b->ctr, b->ranges, n_lambdas);
+                    (void)hipEventRecord(e0, st);
+                    b->newton_mixed(c, d_scenes, n_scenes, n_lambdas, nullptr, st);
+                    const float ms = elapsed();
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = c;
+                    }
+                }
+        b->cfg_mixed = best;
+    }
     // finish kernels: block size, on the converged list of the tuned Newton shapes
     k_reset<<<1, 64, 0, st>>>(b->ctr, b->ranges, n_lambdas);
     for (int K = 1; K <= 3; ++K) run_newton(K, b->cfg[K - 1]);
@@ -1047,22 +1332,56 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
         }
         finish_ms[K - 1] = best_fin;
     }
-    if (report) {
-        for (int K = 1; K <= 3; ++K) {
-            const palz_launch_config& c = b->cfg[K - 1];
-            int32_t* r = report + (K - 1) * 6;
-            r[0] = c.block;
-            r[1] = c.blocks_per_cu;
-            r[2] = c.chunk;
-            r[3] = c.finish_block;
-            r[4] = (int32_t)(newton_ms[K - 1] * 1000.0f);
-            r[5] = (int32_t)(finish_ms[K - 1] * 1000.0f);
+    // the mixed-length occlusion kernel's block size
+    if (b->fused != 0) {
+        float best_ms = 1e30f;
+        for (int fb : blocks) {
+            palz_launch_config c = b->cfg_mixed;
+            c.finish_block = fb;
+            (void)hipEventRecord(e0, st);
+            b->occlude_mixed(c, d_scenes, n_lambdas, st);
+            const float ms = elapsed();
+            if (ms < best_ms) {
+                best_ms = ms;
+                b->cfg_mixed.finish_block = fb;
+            }
         }
     }
+    // mode -1: the whole solve both ways on the tuned shapes, the faster kept
+    if (b->fused == -1) {
+        float best[2] = {1e30f, 1e30f};
+        for (int rep = 0; rep < 2; ++rep)
+            for (int f = 0; f < 2; ++f) {
+                b->auto_fused = f == 1;
+                (void)hipEventRecord(e0, st);
+                rc = b->launch_all(d_scenes, n_scenes, n_lambdas, refine, nullptr, nullptr, st, false);
+                if (rc != PALZ_OK) return rc;
+                best[f] = std::min(best[f], elapsed());
+            }
+        b->auto_fused = best[1] < best[0];
+    }
+    write_report();
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->drop_graph();
     PALZ_LAUNCH_CHECK();
+    if (!b->tune_cache.empty()) {
+        int32_t v[kTuneFields];
+        const palz_launch_config* c4[4] = {&b->cfg[0], &b->cfg[1], &b->cfg[2], &b->cfg_mixed};
+        for (int i = 0; i < 4; ++i) {
+            v[4 * i] = c4[i]->block;
+            v[4 * i + 1] = c4[i]->blocks_per_cu;
+            v[4 * i + 2] = c4[i]->chunk;
+            v[4 * i + 3] = c4[i]->finish_block;
+        }
+        for (int K = 0; K < 3; ++K) {
+            v[16 + 2 * K] = (int32_t)(newton_ms[K] * 1000.0f);
+            v[17 + 2 * K] = (int32_t)(finish_ms[K] * 1000.0f);
+        }
+        v[22] = b->auto_fused ? 1 : 0;
+        if (!tune_cache_append(b->tune_cache, key, v))
+            return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_autotune: cannot append to the tile-shape cache file");
+    }
     return PALZ_OK;
 }
 

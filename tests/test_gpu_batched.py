@@ -304,9 +304,79 @@ def test_autotune_keeps_results(run):
             assert row["newton_us"] > 0
     after, _, _ = run.solve(refine=CA.REFINE_VERTICES)
     assert after.tobytes() == before.tobytes()
+    # mode -1: the autotuner timed whole solves per length and fused and kept one
+    mode, active = run.bs.get_fused()
+    assert mode == -1
+    run.bs.set_fused(0 if active else 1)
+    other_mode, _, _ = run.solve(refine=CA.REFINE_VERTICES)
+    run.bs.set_fused(-1)
+    assert run.bs.get_fused() == (-1, active)
+    assert other_mode.tobytes() == before.tobytes()
     run.bs.set_config(1, 64, 1, 1, 64)
     odd, _, _ = run.solve(refine=CA.REFINE_VERTICES)
     assert odd.tobytes() == before.tobytes()
+
+
+def test_fused_kernels_match_per_length_kernels(run):
+    # one persistent Newton and one occlusion kernel over every chain length:
+    # the same items, results and refined vertices as one kernel per length,
+    # eagerly and in graph replays
+    split, split_ref, split_img = run.solve(refine=CA.REFINE_VERTICES)
+    run.bs.set_fused(True)
+    try:
+        # the fused Newton kernel's register budget: 2 or 3 waves per SIMD
+        for waves in (2, 3):
+            run.bs.set_config(0, 256, 2, waves, 256)
+            for graph in (False, True, True):
+                res, ref, img = run.solve(refine=CA.REFINE_VERTICES, graph=graph)
+                assert res.tobytes() == split.tobytes(), f"waves={waves} graph={graph}"
+                assert ref.tobytes() == split_ref.tobytes(), f"waves={waves} graph={graph}"
+                assert np.allclose(img, split_img, rtol=1e-5, atol=1e-6), f"waves={waves} graph={graph}"
+        st = run.bs.stats()
+        K = run.paths["event"] >> 24
+        for k in (1, 2, 3):
+            assert st[f"K{k}"]["items"] == int((K == k).sum()) * run.L
+    finally:
+        run.bs.set_fused(-1)
+
+
+def test_autotune_tile_shape_cache(run, tmp_path):
+    # the first tuning of a workload appends one line to the cache; a second
+    # solver on the same workload takes its shapes from there without timing
+    cache = tmp_path / "tiles.txt"
+    run.bs.set_tune_cache(str(cache))
+    try:
+        rep = run.bs.autotune(run.d_scenes, 1, run.L, refine=CA.REFINE_VERTICES)
+        lines = cache.read_text().splitlines()
+        assert len(lines) == 1 and lines[0].startswith("palz-tune ")
+        tuned = [run.bs.get_config(k) for k in (1, 2, 3)]
+        other = CA.BatchedSolver(run.g.sorter)
+        other.prepare(run.d_paths, run.n)
+        for k in (1, 2, 3):
+            other.set_config(k, 64, 1, 1, 64)
+        other.set_tune_cache(str(cache))
+        rep2 = other.autotune(run.d_scenes, 1, run.L, refine=CA.REFINE_VERTICES)
+        assert rep2 == rep
+        assert [other.get_config(k) for k in (1, 2, 3)] == tuned
+        assert len(cache.read_text().splitlines()) == 1, "a cache hit appends nothing"
+        # fused mode is another workload key: it tunes and appends
+        other.set_fused(True)
+        other.autotune(run.d_scenes, 1, run.L, refine=CA.REFINE_VERTICES)
+        assert len(cache.read_text().splitlines()) == 2
+        m = other.get_config(0)
+        assert m["block"] in (64, 128, 256) and m["finish_block"] in (64, 128, 256) and m["chunk"] in (2, 3)
+        # a line with shapes out of range is not applied
+        key = lines[0].split(":")[0]
+        cache.write_text(key + ": " + " ".join(["7"] * 23) + "\n")
+        other.set_fused(-1)
+        other.set_config(1, 64, 1, 1, 64)
+        other.autotune(run.d_scenes, 1, run.L, refine=CA.REFINE_VERTICES)
+        assert other.get_config(1)["block"] in (64, 128, 256)
+        assert len(cache.read_text().splitlines()) == 2
+        before, _, _ = run.solve(refine=CA.REFINE_VERTICES)
+    finally:
+        run.bs.set_tune_cache(None)
+    assert before.shape[0] == run.n * run.L
 
 
 def _c4_small():
@@ -337,6 +407,15 @@ def test_multi_scene_batch(make):
     res = _unsort(CA.download_results(d_res, n * L), bs.order(), L)
     st = bs.stats()
     assert sum(st[f"K{k}"]["items"] for k in (1, 2, 3)) + st["skipped"] == n * L
+    # the fused (one kernel over every chain length) solve of the same batch
+    bs.set_fused(True)
+    d_res_fused = DeviceBuffer(n * L * CA.PATH_RESULT.itemsize)
+    batch.clear_images()
+    bs.solve(batch.d_scenes, len(scene_list), L, False, d_res_fused, None, graph=True)
+    assert CA.download_results(d_res_fused, n * L).tobytes() == CA.download_results(d_res, n * L).tobytes()
+    bs.set_fused(False)
+    batch.clear_images()
+    bs.solve(batch.d_scenes, len(scene_list), L, False, d_res, None, graph=True)
     compared = 0
     for i, (s, g) in enumerate(zip(scene_list, batch.scenes)):
         m = (paths["scene"] == i) & ((paths["event"] >> 24) <= 3)
