@@ -346,24 +346,55 @@ struct QR {
 
 namespace {
 
-constexpr int kChunk = 4;
+// Items per workgroup (one wave): their FP64 solutions and pseudo-inverses
+// stay in LDS between the two phases of k_adjoint.
+constexpr int kAdjItems = 64;
 
+// An item's FP64 solution x* and the pseudo-inverse J_x^+ (P x R, row-major)
+// of its constraint Jacobian there.
 template <int K>
-__global__ void __launch_bounds__(64)
-    k_adjoint(palz_scene_desc sc, palz_height_param hp, const palz_path* __restrict__ paths, uint32_t n_paths,
-              const palz_path_result* __restrict__ results, const float* __restrict__ weight,
-              double* __restrict__ grad) {
+struct ItemState {
+    double x[2 * K];
+    double pinv[2 * K * 3 * K];
+};
+
+// The chain's triangles as one-tangent numbers: vertex c of triangle k gets
+// the unit tangent in its height (what 0) or in one shading-normal component
+// (what 1..3); every other input is constant.  Selected per lane without
+// indexing registers by the lane's (k, c).
+template <int K, class G>
+PALZ_HD void seeded_tris(const ChainTri* ct, int ks, int cs, int what, ad::Tri<G>* tg) {
+    using namespace ad;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const v3 v[3] = {ct[k].v0, ct[k].v1, ct[k].v2};
+        const v3 n[3] = {ct[k].s0, ct[k].s1, ct[k].s2};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            tg[k].v[c] = vc<G>(v[c]);
+            tg[k].s[c] = vc<G>(n[c]);
+            const bool me = k == ks && c == cs;
+            tg[k].v[c].y.d[0] = (me && what == 0) ? 1.0 : 0.0;
+            tg[k].s[c].x.d[0] = (me && what == 1) ? 1.0 : 0.0;
+            tg[k].s[c].y.d[0] = (me && what == 2) ? 1.0 : 0.0;
+            tg[k].s[c].z.d[0] = (me && what == 3) ? 1.0 : 0.0;
+        }
+        tg[k].material = ct[k].material;
+    }
+}
+
+// Phase A of one item: the contributing check (stored results, or the FP32
+// forward solve recomputed), three FP64 Newton steps from the FP32 solution,
+// and J_x^+ at x* from J_x's Householder QR.  Returns the chain vertices
+// whose heights are parameters as a bit mask (bit 3k + c), 0 when the item
+// adds nothing.
+template <int K>
+__device__ uint32_t adjoint_prepare(const palz_scene_desc& sc, const palz_height_param& hp, const palz_path& path,
+                                    uint32_t l, uint64_t item, const palz_path_result* __restrict__ results,
+                                    ItemState<K>* st) {
     using namespace ad;
     constexpr int P = 2 * K;
     constexpr int R = 3 * K;
-    const uint32_t L = sc.n_lambdas;
-    const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= (uint64_t)n_paths * L) return;
-    const uint32_t pi = (uint32_t)(item / L), l = (uint32_t)(item % L);
-    const palz_path path = paths[pi];
-    if (ev_size(path.event) != K) return;
-    const float w_img = weight[(size_t)l * sc.n_points + path.pixel];
-    if (w_img == 0.0f) return;
     const uint32_t event = path.event;
     ChainTri ct[K];
 #pragma unroll
@@ -374,7 +405,7 @@ __global__ void __launch_bounds__(64)
     float x32[P];
     if (results) {
         const palz_path_result r = results[item];
-        if (r.status != PALZ_PATH_CONTRIBUTES) return;
+        if (r.status != PALZ_PATH_CONTRIBUTES) return 0u;
 #pragma unroll
         for (int i = 0; i < P; ++i) x32[i] = r.params[i];
     } else {
@@ -384,23 +415,20 @@ __global__ void __launch_bounds__(64)
         int iters = 0;
         if (!solve_constraints<K, 1, float, v3>(x32, mk(sc.p_light), mk(q.p), ct, eta32, event, 32, 1.0e-10f, 1,
                                                 &iters))
-            return;
+            return 0u;
         if (!(0.0f < contributable_throughput<K>(x32, mk(sc.p_light), mk(q.p), ct, event, sc.nodes, sc.tris,
                                                  *sc.root, eta32)))
-            return;
+            return 0u;
     }
-    // the FP64 solution: two Newton steps from the FP32 one
-    Tri<double> td[K];
+    uint32_t mask = 0u;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        td[k].v[0] = vc<double>(ct[k].v0);
-        td[k].v[1] = vc<double>(ct[k].v1);
-        td[k].v[2] = vc<double>(ct[k].v2);
-        td[k].s[0] = vc<double>(ct[k].s0);
-        td[k].s[1] = vc<double>(ct[k].s1);
-        td[k].s[2] = vc<double>(ct[k].s2);
-        td[k].material = ct[k].material;
-    }
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (hp.tri_vertex[3 * path.tris[k] + c] >= 0) mask |= 1u << (3 * k + c);
+    if (mask == 0u) return 0u;
+    // the FP64 solution: three Newton steps from the FP32 one, keeping the
+    // factorisation at x*
     const double eta = (double)eta32;
     using J = Dn<double, P>;
     double x[P];
@@ -417,12 +445,14 @@ __global__ void __launch_bounds__(64)
         Tri<J> tj[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
+            const v3 v[3] = {ct[k].v0, ct[k].v1, ct[k].v2};
+            const v3 n[3] = {ct[k].s0, ct[k].s1, ct[k].s2};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                tj[k].v[c] = {cst<J>(td[k].v[c].x), cst<J>(td[k].v[c].y), cst<J>(td[k].v[c].z)};
-                tj[k].s[c] = {cst<J>(td[k].s[c].x), cst<J>(td[k].s[c].y), cst<J>(td[k].s[c].z)};
+                tj[k].v[c] = vc<J>(v[c]);
+                tj[k].s[c] = vc<J>(n[c]);
             }
-            tj[k].material = td[k].material;
+            tj[k].material = ct[k].material;
         }
         J F[R];
         residual<K, J>(xj, tj, vc<J>(mk(sc.p_light)), vc<J>(mk(q.p)), cst<J>(eta), event, F);
@@ -434,93 +464,147 @@ __global__ void __launch_bounds__(64)
             for (int c = 0; c < P; ++c) qr.a[c * R + r] = F[r].d[c];
         }
         qr.factor();
-        if (step == 2) break;  // keep the factorisation at x*
+        if (step == 2) break;
         double dx[P];
         qr.solve(b, dx);
 #pragma unroll
         for (int i = 0; i < P; ++i) x[i] -= dx[i];
     }
-    // local inputs: per chain triangle whose vertices are parameters, the
-    // height (vertex y) and the shading normal (x, y, z) of each vertex
-    int slot_tri[4 * 3 * K];
-    int slot_vtx[4 * 3 * K];
-    int slot_what[4 * 3 * K];
-    int n_slots = 0;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int t = path.tris[k];
-        for (int c = 0; c < 3; ++c) {
-            const int h = hp.tri_vertex[3 * t + c];
-            if (h < 0) continue;
-            for (int what = 0; what < 4; ++what) {
-                slot_tri[n_slots] = k;
-                slot_vtx[n_slots] = c;
-                slot_what[n_slots] = what;
-                ++n_slots;
-            }
+    for (int i = 0; i < P; ++i) st->x[i] = x[i];
+    // J_x^+ column by column: the least-squares solutions of J_x y = e_r
+    for (int r = 0; r < R; ++r) {
+        double e[R], y[P];
+#pragma unroll
+        for (int i = 0; i < R; ++i) e[i] = i == r ? 1.0 : 0.0;
+        qr.solve(e, y);
+#pragma unroll
+        for (int i = 0; i < P; ++i) st->pinv[i * R + r] = y[i];
+    }
+    return mask;
+}
+
+// Phase B of one (item, chain vertex, input) triple: with the input's unit
+// tangent, J_q = dF/dq by one forward pass of the residual at x*, dx*/dq =
+// -J_x^+ J_q, then the total dc/dq by one forward pass of the contribution
+// carrying both tangents.  Returns the weighted derivative.
+template <int K>
+__device__ double adjoint_input(const palz_scene_desc& sc, const palz_path& path, uint32_t l, float w_img,
+                                const ItemState<K>& st, int ks, int cs, int what) {
+    using namespace ad;
+    constexpr int P = 2 * K;
+    constexpr int R = 3 * K;
+    using G = Dn<double, 1>;
+    const uint32_t event = path.event;
+    ChainTri ct[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) ct[k] = load_chain_tri(sc.tris, sc.attribs, path.tris[k]);
+    const palz_first_diffuse q = sc.points[path.pixel];
+    const float lam = sc.lambdas[l];
+    const G eta = cst<G>((double)(sc.cauchy_a + sc.cauchy_b / (lam * lam)));
+    Tri<G> tg[K];
+    seeded_tris<K, G>(ct, ks, cs, what, tg);
+    const V3<G> pb = vc<G>(mk(sc.p_light)), pe = vc<G>(mk(q.p)), ng = vc<G>(mk(q.ng));
+    G xg[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) xg[i] = cst<G>(st.x[i]);
+    G F[R];
+    residual<K, G>(xg, tg, pb, pe, eta, event, F);
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) s += st.pinv[i * R + r] * F[r].d[0];
+        xg[i].d[0] = -s;
+    }
+    const G c = contribution<K, G>(xg, tg, pb, pe, ng, eta, event, sc.light_intensity, sc.radiance_clamp);
+    return (double)w_img * c.d[0];
+}
+
+// d(loss)/d(heights) of the contributing items of chain length K.  One wave
+// per kAdjItems consecutive items: phase A prepares each item in its own lane
+// (x*, J_x^+ into LDS); phase B gives every (item, parameter vertex, input)
+// triple a lane of its own, four lanes per vertex: the height and the three
+// shading-normal components, each a single-tangent forward pass.  The four
+// lanes of a vertex then combine the normal's chain rule and one of them adds
+// the vertex's share into the heights with FP64 atomics.
+template <int K>
+__global__ void __launch_bounds__(kAdjItems)
+    k_adjoint(palz_scene_desc sc, palz_height_param hp, const palz_path* __restrict__ paths, uint32_t n_paths,
+              const palz_path_result* __restrict__ results, const float* __restrict__ weight,
+              double* __restrict__ grad) {
+    __shared__ ItemState<K> s_state[kAdjItems];
+    __shared__ uint16_t s_unit[kAdjItems * 3 * K];  // lane << 4 | chain vertex (3k + c)
+    const uint32_t L = sc.n_lambdas;
+    const uint32_t lane = threadIdx.x;
+    const uint64_t n_items = (uint64_t)n_paths * L;
+    const uint64_t base = (uint64_t)blockIdx.x * kAdjItems;
+    // phase A
+    uint32_t mask = 0u;
+    {
+        const uint64_t item = base + lane;
+        if (item < n_items) {
+            const uint32_t pi = (uint32_t)(item / L), l = (uint32_t)(item % L);
+            const palz_path path = paths[pi];
+            if (ev_size(path.event) == K && weight[(size_t)l * sc.n_points + path.pixel] != 0.0f)
+                mask = adjoint_prepare<K>(sc, hp, path, l, item, results, &s_state[lane]);
         }
     }
-    double gq[4 * 3 * K];
-    using G = Dn<double, kChunk>;
-    for (int s0 = 0; s0 < n_slots; s0 += kChunk) {
-        // inputs with their tangents
-        Tri<G> tg[K];
+    // list the (item, vertex) units in lane order
+    const uint32_t cnt = (uint32_t)__popc(mask);
+    uint32_t incl = cnt;
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                tg[k].v[c] = {cst<G>(td[k].v[c].x), cst<G>(td[k].v[c].y), cst<G>(td[k].v[c].z)};
-                tg[k].s[c] = {cst<G>(td[k].s[c].x), cst<G>(td[k].s[c].y), cst<G>(td[k].s[c].z)};
-            }
-            tg[k].material = td[k].material;
-        }
-        for (int t = 0; t < kChunk && s0 + t < n_slots; ++t) {
-            const int k = slot_tri[s0 + t], c = slot_vtx[s0 + t], what = slot_what[s0 + t];
-            if (what == 0) tg[k].v[c].y.d[t] = 1.0;
-            if (what == 1) tg[k].s[c].x.d[t] = 1.0;
-            if (what == 2) tg[k].s[c].y.d[t] = 1.0;
-            if (what == 3) tg[k].s[c].z.d[t] = 1.0;
-        }
-        const V3<G> pb = vc<G>(mk(sc.p_light)), pe = vc<G>(mk(q.p)), ng = vc<G>(mk(q.ng));
-        const G eta_g = cst<G>(eta);
-        // J_q, then dx*/dq = -J_x^+ J_q
-        G xg[P];
-#pragma unroll
-        for (int i = 0; i < P; ++i) xg[i] = cst<G>(x[i]);
-        G F[R];
-        residual<K, G>(xg, tg, pb, pe, eta_g, event, F);
-#pragma unroll
-        for (int t = 0; t < kChunk; ++t) {
-            double bq[R], dx[P];
-#pragma unroll
-            for (int r = 0; r < R; ++r) bq[r] = F[r].d[t];
-            qr.solve(bq, dx);
-#pragma unroll
-            for (int i = 0; i < P; ++i) xg[i].d[t] = -dx[i];
-        }
-        const G c = contribution<K, G>(xg, tg, pb, pe, ng, eta_g, event, sc.light_intensity, sc.radiance_clamp);
-        for (int t = 0; t < kChunk && s0 + t < n_slots; ++t) gq[s0 + t] = (double)w_img * c.d[t];
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, o, 64);
+        if (lane >= (uint32_t)o) incl += y;
     }
-    // into the heights: y directly; the normal through n = m / |m|,
-    // m = (-dh/dx, 1, -dh/dz) from central differences of the heights
-    for (int s = 0; s < n_slots; s += 4) {
-        const int k = slot_tri[s], c = slot_vtx[s];
-        const int h = hp.tri_vertex[3 * path.tris[k] + c];
-        atomicAdd(grad + h, gq[s]);
-        const v3 n = c == 0 ? ct[k].s0 : (c == 1 ? ct[k].s1 : ct[k].s2);
-        const double nx = n.x, ny = n.y, nz = n.z;
-        const double gx = gq[s + 1], gy = gq[s + 2], gz = gq[s + 3];
-        // dL/dm = (I - n n^T) g / |m|, |m| = 1 / |n_y|
-        const double ndg = nx * gx + ny * gy + nz * gz;
-        const double inv_m = fabs(ny);
-        const double dmx = (gx - nx * ndg) * inv_m;
-        const double dmz = (gz - nz * ndg) * inv_m;
-        for (int j = 0; j < 4; ++j) {
-            const int nb = hp.normal_nb[4 * h + j];
-            if (nb < 0) continue;
-            // m_x = -hx, m_z = -hz
-            const double v = -dmx * hp.normal_coef[8 * h + 2 * j] - dmz * hp.normal_coef[8 * h + 2 * j + 1];
-            atomicAdd(grad + nb, v);
+    const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+    uint32_t pos = incl - cnt;
+    for (uint32_t m = mask; m != 0u; m &= m - 1u) s_unit[pos++] = (uint16_t)(lane << 4 | (uint32_t)__ffs(m) - 1u);
+    __syncthreads();
+    // phase B: four lanes per unit
+    for (uint32_t t0 = 0; t0 < 4u * total; t0 += 64u) {
+        const uint32_t t = t0 + lane;
+        const bool active = t < 4u * total;
+        const uint32_t unit = active ? s_unit[t >> 2] : 0u;
+        const uint32_t il = unit >> 4, v = unit & 15u;
+        const int ks = (int)(v / 3u), cs = (int)(v % 3u), what = (int)(t & 3u);
+        double g = 0.0, dmx = 0.0, dmz = 0.0;
+        int h = -1;
+        palz_path path;
+        if (active) {
+            const uint64_t item = base + il;
+            const uint32_t pi = (uint32_t)(item / L), l = (uint32_t)(item % L);
+            path = paths[pi];
+            const float w_img = weight[(size_t)l * sc.n_points + path.pixel];
+            g = adjoint_input<K>(sc, path, l, w_img, s_state[il], ks, cs, what);
+            h = hp.tri_vertex[3 * path.tris[ks] + cs];
+            if (what > 0) {
+                // the normal n = m / |m|, m = (-dh/dx, 1, -dh/dz): this
+                // component's share of dL/dm = (I - n n^T) g / |m|, |m| = 1 / |n_y|
+                const palz_triangle_attrib a = sc.attribs[path.tris[ks]];
+                const double n3[3] = {a.shading_normals[cs].x, a.shading_normals[cs].y, a.shading_normals[cs].z};
+                const int comp = what - 1;
+                const double inv_m = fabs(n3[1]);
+                dmx = ((comp == 0 ? g : 0.0) - n3[0] * n3[comp] * g) * inv_m;
+                dmz = ((comp == 2 ? g : 0.0) - n3[2] * n3[comp] * g) * inv_m;
+            }
+        }
+        // the vertex's four lanes are adjacent: sum the normal's shares
+#pragma unroll
+        for (int o = 1; o < 4; o <<= 1) {
+            dmx += __shfl_xor(dmx, o, 64);
+            dmz += __shfl_xor(dmz, o, 64);
+        }
+        if (active && what == 0) {
+            atomicAdd(grad + h, g);
+            for (int j = 0; j < 4; ++j) {
+                const int nb = hp.normal_nb[4 * h + j];
+                if (nb < 0) continue;
+                // m_x = -hx, m_z = -hz
+                const double d = -dmx * hp.normal_coef[8 * h + 2 * j] - dmz * hp.normal_coef[8 * h + 2 * j + 1];
+                atomicAdd(grad + nb, d);
+            }
         }
     }
 }
@@ -540,11 +624,11 @@ int palz_adjoint_irradiance(const palz_scene_desc* scene, const palz_height_para
     const uint64_t items = (uint64_t)n_paths * scene->n_lambdas;
     if (items == 0) return PALZ_OK;
     if (items >= (1ull << 31)) return fail(PALZ_ERR_CAPACITY, "palz_adjoint_irradiance: more than 2^31 items");
-    const uint32_t blocks = (uint32_t)((items + 63) / 64);
+    const uint32_t blocks = (uint32_t)((items + kAdjItems - 1) / kAdjItems);
     hipStream_t st = (hipStream_t)stream;
-    k_adjoint<1><<<blocks, 64, 0, st>>>(*scene, *hp, d_paths, n_paths, d_results, d_weight, d_grad);
-    k_adjoint<2><<<blocks, 64, 0, st>>>(*scene, *hp, d_paths, n_paths, d_results, d_weight, d_grad);
-    k_adjoint<3><<<blocks, 64, 0, st>>>(*scene, *hp, d_paths, n_paths, d_results, d_weight, d_grad);
+    k_adjoint<1><<<blocks, kAdjItems, 0, st>>>(*scene, *hp, d_paths, n_paths, d_results, d_weight, d_grad);
+    k_adjoint<2><<<blocks, kAdjItems, 0, st>>>(*scene, *hp, d_paths, n_paths, d_results, d_weight, d_grad);
+    k_adjoint<3><<<blocks, kAdjItems, 0, st>>>(*scene, *hp, d_paths, n_paths, d_results, d_weight, d_grad);
     PALZ_LAUNCH_CHECK();
     return PALZ_OK;
 }
