@@ -7,12 +7,15 @@
 // solveConstraints (pk.h:477-590).  By the implicit-function theorem
 //     dx*/dq = -J_x^+ J_q,   J_x = dF/dx (3K x 2K),  J_q = dF/dq,
 // so  dc/dq = dc/dq|_x + dc/dx dx*/dq.  Per item the kernel refines x* in FP64,
-// forms J_x with one multi-tangent pass and its Householder QR, then for
-// chunks of 4 local inputs: J_q by a forward pass, dx*/dq by the QR, and the
-// total dc/dq by one forward pass of the contribution (throughput, dA/dw with
-// its own two light-direction tangents nested inside, the receiver cosine and
-// the clamp: pk.h:760-930, main_gpu.cu:458-471).  The item's weight dL/dI
-// times dc/dq is then pushed through the heights' normals (central
+// forms J_x with one multi-tangent pass and its Householder QR, and keeps x*
+// and J_x^+ in LDS.  Then every local input has a lane of its own: J_q e by a
+// one-tangent forward pass of the residual, dx*/dq e = -J_x^+ J_q e, and the
+// total dc/dq e by one forward pass of the contribution carrying both
+// (throughput, dA/dw with its own two light-direction tangents nested inside,
+// the receiver cosine and the clamp: pk.h:760-930, main_gpu.cu:458-471).  One
+// tangent per lane keeps the nested numbers small enough for registers: the
+// state of 4 tangents in one lane spilled 1.3-2.2k VGPRs.  The item's weight
+// dL/dI times dc/dq is then pushed through the heights' normals (central
 // differences, normalised) into the height gradient with FP64 atomics.
 // Visibility, validity and the tuple set are those of the forward solve.
 #include "optics.hpp"
