@@ -202,7 +202,7 @@ def bench_c2(args) -> dict:
     d_res = DeviceBuffer(max(int(ranges[5]) * L, 1) * CA.PATH_RESULT.itemsize)
 
     if args.launch_config:
-        for K, c in enumerate(json.loads(args.launch_config), start=1):
+        for K, c in enumerate(json.loads(args.launch_config)[:3], start=1):
             bs.set_config(K, *c)
         tune = [bs.get_config(K) for K in (1, 2, 3)]
         log(f"[bench] c2: launch shapes {tune}")
@@ -577,7 +577,17 @@ def bench_c4(args) -> dict:
     bs.prepare(batch.d_paths, n)
     ranges = bs.ranges()
     assert ranges[3] == sum(x[4] for x in plain)
-    tune = bs.autotune(batch.d_scenes, len(scene_list), L, refine=CA.REFINE_VERTICES)
+    if args.launch_config:
+        # fixed shapes (a profiling run): per length and, when given, the fused kernels'
+        cfg = json.loads(args.launch_config)
+        for K, c in enumerate(cfg[:3], start=1):
+            bs.set_config(K, *c)
+        if len(cfg) > 3:
+            bs.set_config(0, *cfg[3])
+        bs.set_fused(1 if len(cfg) > 3 else 0)
+        tune = [bs.get_config(K) for K in (1, 2, 3)]
+    else:
+        tune = bs.autotune(batch.d_scenes, len(scene_list), L, refine=CA.REFINE_VERTICES)
     fused_chosen = bs.get_fused()[1]
     d_res = DeviceBuffer(max(n * L, 1) * CA.PATH_RESULT.itemsize)
     t0, t1 = Event(), Event()
@@ -639,7 +649,11 @@ def bench_fused(args, bs, d_scenes, n_scenes, L, d_res, st, graph: bool, check: 
     ref = CA.download_results(d_res, int(bs.ranges()[5]) * L) if check else None
     bs.set_fused(True)
     try:
-        bs.autotune(d_scenes, n_scenes, L, refine=CA.REFINE_VERTICES)
+        cfg = json.loads(args.launch_config) if args.launch_config else []
+        if len(cfg) > 3:
+            bs.set_config(0, *cfg[3])
+        else:
+            bs.autotune(d_scenes, n_scenes, L, refine=CA.REFINE_VERTICES)
         shape = bs.get_config(0)
         d_out = DeviceBuffer(d_res.nbytes) if check else d_res
         for _ in range(args.warmup + 1):
@@ -938,7 +952,8 @@ def main() -> int:
     ap.add_argument("--vertex-sample", type=int, default=512)
     ap.add_argument("--c4-scenes", type=int, default=256)
     ap.add_argument("--launch-config", default=None,
-                    help="skip the c2 autotuner: JSON [[block, blocks_per_cu, chunk, finish_block]] for K = 1..3")
+                    help="skip the c2 autotuner: JSON [[block, blocks_per_cu, chunk, finish_block]] for K = 1..3 "
+                         "and, optionally, the fused kernels' [block, blocks_per_cu, waves per SIMD, occlusion block]")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
