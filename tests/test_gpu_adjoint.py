@@ -158,19 +158,24 @@ def test_adjoint_full_size_matches_fp64_finite_differences():
         own = np.nonzero((sorted_paths["event"] >> 24) == st.K)[0]
         glob = own[st.item[sel] // r.L] * r.L + st.item[sel] % r.L
         c32 = res["contribution"][glob].astype(np.float64)
+        # an FP64 re-solve that ends in total internal reflection has no
+        # finite value: counted apart, the rest compared
+        fin = np.isfinite(c64)
+        c64, c32 = c64[fin], c32[fin]
         close = np.isclose(c64, c32, rtol=1e-3, atol=1e-6)
         far = ~close
         rel = np.abs(c64 - c32) / np.maximum(np.abs(c64), 1e-30)
         share = np.abs(c64[far]).sum() / np.abs(c64).sum()
         sums = abs(c64.sum() - c32.sum()) / abs(c64.sum())
-        print(f"K={st.K}: {sel.size} sampled items, {int(far.sum())} beyond rtol 1e-3 (largest "
-              f"{rel.max():.2e}, {share:.2e} of the sample's irradiance); sums differ by {sums:.2e}", flush=True)
+        print(f"K={st.K}: {sel.size} sampled items, {int((~fin).sum())} without a finite FP64 value, "
+              f"{int(far.sum())} beyond rtol 1e-3 (largest {rel.max():.2e}, {share:.2e} of the sample's "
+              f"irradiance); sums differ by {sums:.2e}", flush=True)
         # K <= 2: the FP32 roots are the FP64 roots to round-off, up to a few
         # items near a caustic fold.  K = 3 (T-R-T through the slab's bottom)
         # is ill-conditioned: FP32 Newton stops at an FP64 residual of ~5e-6,
         # and the FP64 re-solve moves some chains' parameters by ~5e-4.
         if st.K <= 2:
-            assert far.sum() <= 1e-3 * sel.size and sums <= 1e-4
+            assert fin.all() and far.sum() <= 1e-3 * sel.size and sums <= 1e-4
     live = np.nonzero(np.abs(grad) > 0)[0]
     assert live.size > 1000
     base = r.image
