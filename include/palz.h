@@ -603,10 +603,14 @@ typedef struct {
 
 int palz_batched_create(palz_batched** out);
 int palz_batched_destroy(palz_batched* b);
-/* Stable partition of n_paths paths by chain length into the batch's own
- * buffer.  `d_count` (optional, device) replaces n_paths, which is then the
- * capacity.  After the call, palz_batched_order gives, for every batch slot,
- * the index of the path in d_paths. */
+/* Sorts n_paths paths into the batch's own buffer by chain length, then by
+ * scene, then by first triangle (its index shifted down so that the key fits
+ * 32 bits), keeping the input order of equal keys: each chain length's items
+ * come grouped by scene and by region of the scene, so that waves hold one
+ * scene's neighbouring chains and each XCD's share of a list covers one part
+ * of a scene.  `d_count` (optional, device) replaces n_paths, which is then
+ * the capacity.  After the call, palz_batched_order gives, for every batch
+ * slot, the index of the path in d_paths. */
 int palz_batched_prepare(palz_batched* b, const palz_path* d_paths, uint32_t n_paths, const uint32_t* d_count,
                          palz_sorter* sorter, void* stream);
 int palz_batched_order(palz_batched* b, const uint32_t** d_order, const palz_path** d_sorted_paths,

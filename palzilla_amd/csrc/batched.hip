@@ -4,9 +4,9 @@
 // reads the path count back to the host before each launch (pk.h:1480-1586,
 // :1502-1503); inside a launch a wave runs as many Newton iterations as its
 // slowest lane.  Here:
-//   * paths are partitioned by chain length on the device (stable, so each
-//     length keeps its tuple-coherent order), and every count stays on the
-//     device;
+//   * paths are sorted on the device by chain length, scene and first
+//     triangle (stable, so a tuple's paths stay together), so that waves see
+//     one scene's neighbouring chains, and every count stays on the device;
 //   * per chain length one persistent kernel keeps every lane busy: lanes
 //     whose chain converged, left the triangle or ran out of iterations take
 //     the next (path, wavelength) item from a per-wave chunk of a global work
@@ -62,13 +62,54 @@ __global__ void k_reset(Counters* c, const uint32_t* __restrict__ ranges, uint32
     if (threadIdx.x < 3) c->conv_paths[threadIdx.x] = 0ull;
 }
 
-// words[i] = chain length | i << 32
+__global__ void k_reset_bounds(uint32_t* ranges) {
+    if (threadIdx.x == 0) {
+        ranges[6] = 0u;
+        ranges[7] = 0u;
+    }
+}
+
+// The largest scene index and first-triangle index over the paths (ranges[6], [7]).
+__global__ void __launch_bounds__(256) k_key_bounds(const palz_path* __restrict__ paths, uint32_t n,
+                                                    const uint32_t* __restrict__ d_count, uint32_t* __restrict__ ranges) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t m = d_count ? min(n, *d_count) : n;
+    uint32_t s = 0u, t = 0u;
+    if (i < m) {
+        s = paths[i].scene;
+        t = (uint32_t)paths[i].tris[0];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s = max(s, (uint32_t)__shfl_xor((int)s, o, 64));
+        t = max(t, (uint32_t)__shfl_xor((int)t, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(ranges + 6, s);
+        atomicMax(ranges + 7, t);
+    }
+}
+
+__device__ __forceinline__ uint32_t bits_for(uint32_t v) { return v ? 32u - (uint32_t)__clz((int)v) : 0u; }
+
+// words[i] = key | i << 32, key = chain length (bits 29-31) | scene | first
+// triangle (the low 29 bits, the triangle index shifted down to fit)
 __global__ void __launch_bounds__(256) k_size_words(const palz_path* __restrict__ paths, uint32_t n,
-                                                    const uint32_t* __restrict__ d_count, uint64_t* __restrict__ words) {
+                                                    const uint32_t* __restrict__ d_count,
+                                                    const uint32_t* __restrict__ ranges, uint64_t* __restrict__ words) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t m = d_count ? min(n, *d_count) : n;
     if (i >= m) return;
-    words[i] = (uint64_t)min(ev_size(paths[i].event), 7) | ((uint64_t)i << 32);
+    const palz_path p = paths[i];
+    const uint32_t sb = min(bits_for(ranges[6]), 29u);
+    const uint32_t avail = 29u - sb;
+    const uint32_t tb = bits_for(ranges[7]);
+    const uint32_t shift = tb > avail ? tb - avail : 0u;
+    const uint32_t tri = shift >= 32u ? 0u : ((uint32_t)p.tris[0] >> shift);
+    const uint32_t scene = p.scene & ((1u << sb) - 1u);
+    const uint32_t low = avail ? ((scene << avail) | (tri & ((1u << avail) - 1u))) : scene;
+    const uint32_t key = (uint32_t)min(ev_size(p.event), 7) << 29 | low;
+    words[i] = (uint64_t)key | ((uint64_t)i << 32);
 }
 
 __global__ void __launch_bounds__(256) k_gather_sorted(const uint64_t* __restrict__ words,
@@ -99,7 +140,7 @@ __global__ void k_ranges(const uint64_t* __restrict__ words, const uint32_t* __r
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        if ((uint32_t)(words[mid] & 7u) <= (uint32_t)k)
+        if (((uint32_t)words[mid] >> 29) <= (uint32_t)k)
             lo = mid + 1;
         else
             hi = mid;
@@ -1024,11 +1065,13 @@ int palz_batched_prepare(palz_batched* b, const palz_path* d_paths, uint32_t n_p
         PALZ_LAUNCH_CHECK();
         return PALZ_OK;
     }
-    k_size_words<<<nb, 256, 0, st>>>(d_paths, n_paths, d_count, b->words);
+    k_reset_bounds<<<1, 64, 0, st>>>(b->ranges);
+    k_key_bounds<<<nb, 256, 0, st>>>(d_paths, n_paths, d_count, b->ranges);
+    k_size_words<<<nb, 256, 0, st>>>(d_paths, n_paths, d_count, b->ranges, b->words);
     const SortScratch& s = sorter->scratch;
     launch_sort_prepare(s, nullptr, st);
-    launch_sort_count(s, b->words, n_paths, 0, 3, st, d_count);
-    const int npass = launch_sort_passes(s, b->words, b->words_tmp, nullptr, nullptr, n_paths, 0, 3, st, nullptr,
+    launch_sort_count(s, b->words, n_paths, 0, 32, st, d_count);
+    const int npass = launch_sort_passes(s, b->words, b->words_tmp, nullptr, nullptr, n_paths, 0, 32, st, nullptr,
                                          d_count);
     const uint64_t* sorted = (npass & 1) ? b->words_tmp : b->words;
     k_gather_sorted<<<nb, 256, 0, st>>>(sorted, d_paths, n_paths, d_count, b->paths, b->order, b->ranges + 5);
@@ -1201,6 +1244,7 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
     PALZ_HIP(hipMemcpy(ranges, b->ranges, sizeof(ranges), hipMemcpyDeviceToHost));
     const TuneKey key = tune_key(b->cus, ranges, n_scenes, n_lambdas, refine, b->cull, b->fused);
     float newton_ms[3] = {0.0f, 0.0f, 0.0f}, finish_ms[3] = {0.0f, 0.0f, 0.0f};
+    int32_t times_us[6] = {0, 0, 0, 0, 0, 0};  // Newton and finish per chain length, as reported and cached
     auto write_report = [&]() {
         if (!report) return;
         for (int K = 1; K <= 3; ++K) {
@@ -1210,8 +1254,8 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
             r[1] = c.blocks_per_cu;
             r[2] = c.chunk;
             r[3] = c.finish_block;
-            r[4] = (int32_t)(newton_ms[K - 1] * 1000.0f);
-            r[5] = (int32_t)(finish_ms[K - 1] * 1000.0f);
+            r[4] = times_us[2 * (K - 1)];
+            r[5] = times_us[2 * (K - 1) + 1];
         }
     };
     if (!b->tune_cache.empty()) {
@@ -1225,10 +1269,7 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
                 std::copy(c, c + 3, b->cfg);
                 b->cfg_mixed = c[3];
                 b->auto_fused = v[22] == 1;
-                for (int K = 0; K < 3; ++K) {
-                    newton_ms[K] = (float)v[16 + 2 * K] / 1000.0f;
-                    finish_ms[K] = (float)v[17 + 2 * K] / 1000.0f;
-                }
+                for (int i = 0; i < 6; ++i) times_us[i] = v[16 + i];
                 write_report();
                 b->drop_graph();
                 return PALZ_OK;
@@ -1360,6 +1401,10 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
             }
         b->auto_fused = best[1] < best[0];
     }
+    for (int K = 0; K < 3; ++K) {
+        times_us[2 * K] = (int32_t)(newton_ms[K] * 1000.0f);
+        times_us[2 * K + 1] = (int32_t)(finish_ms[K] * 1000.0f);
+    }
     write_report();
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1374,10 +1419,7 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
             v[4 * i + 2] = c4[i]->chunk;
             v[4 * i + 3] = c4[i]->finish_block;
         }
-        for (int K = 0; K < 3; ++K) {
-            v[16 + 2 * K] = (int32_t)(newton_ms[K] * 1000.0f);
-            v[17 + 2 * K] = (int32_t)(finish_ms[K] * 1000.0f);
-        }
+        for (int i = 0; i < 6; ++i) v[16 + i] = times_us[i];
         v[22] = b->auto_fused ? 1 : 0;
         if (!tune_cache_append(b->tune_cache, key, v))
             return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_autotune: cannot append to the tile-shape cache file");

@@ -113,9 +113,15 @@ def test_partition_by_chain_length(run):
     sp = run.bs.sorted_paths()
     assert sp.tobytes() == run.paths[run.order].tobytes()
     ks = sp["event"] >> 24
+    assert np.all(np.diff(ks.astype(np.int64)) >= 0), "chain lengths must come in order"
+    # within a chain length: by (scene, first triangle), ties in input order
+    key = sp["scene"].astype(np.int64) << 32 | sp["tris"][:, 0].astype(np.int64)
     for k in range(1, 5):
-        idx = run.order[ks == k]
-        assert np.all(np.diff(idx.astype(np.int64)) > 0), "partition must be stable"
+        m = ks == k
+        kk, idx = key[m], run.order[m].astype(np.int64)
+        step = np.diff(kk)
+        assert np.all(step >= 0), "paths of one chain length must be ordered by scene and first triangle"
+        assert np.all(np.diff(idx)[step == 0] > 0), "equal keys must keep their input order"
 
 
 def test_batched_matches_oracle_bit_exact(run):
