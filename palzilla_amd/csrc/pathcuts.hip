@@ -49,8 +49,9 @@ __global__ void k_write_count(uint32_t* count, const unsigned long long* total) 
 }  // namespace palz
 
 struct palz_path_cuts {
-    uint32_t cap_entries = 0;
+    uint32_t cap_list[2] = {0, 0};
     palz::cuts::CutEntry* list[2] = {nullptr, nullptr};  // pending tuples, ping-pong
+    uint32_t cap_counts = 0;
     uint32_t* counts = nullptr;
     uint32_t* scan_sums = nullptr;
     size_t cap_stack = 0;  // words
@@ -79,18 +80,34 @@ struct palz_path_cuts {
         return PALZ_OK;
     }
 
-    int ensure_lists(uint32_t n_entries) {
-        if (n_entries > cap_entries) {
-            for (void* p : {(void*)list[0], (void*)list[1], (void*)counts, (void*)scan_sums})
+    // Buffers grow to the next power of two of what a level needs (contents
+    // are not kept: a list is grown only before it is written).
+    static uint32_t grown(uint32_t n) {
+        uint32_t c = 1u << 16;
+        while (c < n) c <<= 1;
+        return c;
+    }
+    int ensure_list(int i, uint32_t n_entries) {
+        if (n_entries > cap_list[i]) {
+            if (list[i]) (void)hipFree(list[i]);
+            list[i] = nullptr;
+            cap_list[i] = 0;
+            const uint32_t c = grown(n_entries);
+            PALZ_HIP(hipMalloc(&list[i], sizeof(palz::cuts::CutEntry) * c));
+            cap_list[i] = c;
+        }
+        return PALZ_OK;
+    }
+    int ensure_counts(uint32_t n_entries) {
+        if (n_entries > cap_counts) {
+            for (void* p : {(void*)counts, (void*)scan_sums})
                 if (p) (void)hipFree(p);
-            list[0] = list[1] = nullptr;
             counts = scan_sums = nullptr;
-            cap_entries = 0;
-            PALZ_HIP(hipMalloc(&list[0], sizeof(palz::cuts::CutEntry) * n_entries));
-            PALZ_HIP(hipMalloc(&list[1], sizeof(palz::cuts::CutEntry) * n_entries));
-            PALZ_HIP(hipMalloc(&counts, sizeof(uint32_t) * n_entries));
-            PALZ_HIP(hipMalloc(&scan_sums, sizeof(uint32_t) * palz::scan_scratch_entries(n_entries)));
-            cap_entries = n_entries;
+            cap_counts = 0;
+            const uint32_t c = grown(n_entries);
+            PALZ_HIP(hipMalloc(&counts, sizeof(uint32_t) * c));
+            PALZ_HIP(hipMalloc(&scan_sums, sizeof(uint32_t) * palz::scan_scratch_entries(c)));
+            cap_counts = c;
         }
         if (!total) PALZ_HIP(hipMalloc(&total, sizeof(unsigned long long)));
         if (!scalars) PALZ_HIP(hipMalloc(&scalars, sizeof(uint32_t) * 4));
@@ -147,8 +164,10 @@ int palz_path_cuts_enumerate(palz_path_cuts* pc, const palz_scene_desc* scene, c
         return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_path_cuts_enumerate: incomplete scene");
     if (scene->n_points > kCutsListCap)
         return fail(PALZ_ERR_CAPACITY, "palz_path_cuts_enumerate: more than 2^24 receiver points");
-    const uint32_t cap = std::max(scene->n_points, kCutsListCap / 8);
-    int rc = pc->ensure_lists(cap);
+    // the seed list, and the counts with the call's scalars and total (whose
+    // addresses go into the kernel arguments below)
+    int rc = pc->ensure_list(0, scene->n_points);
+    if (rc == PALZ_OK) rc = pc->ensure_counts(scene->n_points);
     if (rc != PALZ_OK) return rc;
     CutsArgs a;
     a.tris = scene->tris;
@@ -166,20 +185,24 @@ int palz_path_cuts_enumerate(palz_path_cuts* pc, const palz_scene_desc* scene, c
     a.ly = scene->p_light.y;
     a.lz = scene->p_light.z;
     a.stack = nullptr;  // set once the walkers are counted
-    a.counts = pc->counts;
+    a.counts = nullptr;  // set per level, as the counts buffer grows
     a.out = d_paths;
     a.capacity = capacity;
     a.status = d_status;
     a.total = pc->total;
     a.words = nullptr;
 
-    // Breadth-first expansion until the list holds enough tuples to fill the
-    // device: one host read of the level's total per level.
+    // Breadth-first expansion until the list holds enough tuples that the
+    // walks below are short: one host read of the level's total per level.
+    // Every lane of an expansion tests one tuple's children, so the lanes stay
+    // busy; the walks' lanes diverge (each walks a subtree of its own).
     int cur = 0;
     uint32_t n = scene->n_points;
     k_seed<<<div_up(n, 256u), 256, 0, st>>>(pc->list[0], n, scene->root);
     PALZ_LAUNCH_CHECK();
     for (int level = 0; level < kCutsMaxLevels && n > 0 && n < kCutsExpandTarget; ++level) {
+        if ((rc = pc->ensure_counts(n)) != PALZ_OK) return rc;
+        a.counts = pc->counts;
         k_write_zero<<<1, 1, 0, st>>>(pc->scalars);
         if (K == 1) launch_expand<1>(a, pc->list[cur], n, pc->counts, nullptr, pc->scalars, st, false);
         if (K == 2) launch_expand<2>(a, pc->list[cur], n, pc->counts, nullptr, pc->scalars, st, false);
@@ -190,7 +213,8 @@ int palz_path_cuts_enumerate(palz_path_cuts* pc, const palz_scene_desc* scene, c
         uint32_t h[2];
         PALZ_HIP(hipMemcpyAsync(h, pc->scalars, sizeof(h), hipMemcpyDeviceToHost, st));
         PALZ_HIP(hipStreamSynchronize(st));
-        if (!h[0] || h[1] > cap) break;  // nothing left to cut, or the list would not fit
+        if (!h[0] || h[1] > kCutsListMax) break;  // nothing left to cut, or the list would not fit
+        if ((rc = pc->ensure_list(cur ^ 1, h[1])) != PALZ_OK) return rc;
         if (K == 1) launch_expand<1>(a, pc->list[cur], n, pc->counts, pc->list[cur ^ 1], pc->scalars, st, true);
         if (K == 2) launch_expand<2>(a, pc->list[cur], n, pc->counts, pc->list[cur ^ 1], pc->scalars, st, true);
         if (K == 3) launch_expand<3>(a, pc->list[cur], n, pc->counts, pc->list[cur ^ 1], pc->scalars, st, true);
@@ -205,6 +229,8 @@ int palz_path_cuts_enumerate(palz_path_cuts* pc, const palz_scene_desc* scene, c
     }
 
     // walk every listed tuple's subtree: count, scan, write
+    if ((rc = pc->ensure_counts(n)) != PALZ_OK) return rc;
+    a.counts = pc->counts;
     a.n_threads = std::min(div_up(n, (uint32_t)kCutsThreads) * kCutsThreads, kCutsMaxThreads);
     rc = pc->ensure_stack((size_t)kCutsStack * K * a.n_threads);
     if (rc != PALZ_OK) return rc;

@@ -13,7 +13,8 @@ namespace cuts {
 constexpr int kCutsThreads = 128;
 constexpr uint32_t kCutsStack = 192;                      // stack entries per thread
 constexpr uint32_t kCutsMaxThreads = 2048u * kCutsThreads;  // resident walkers
-constexpr uint32_t kCutsExpandTarget = 1u << 20;  // listed tuples: work for every walker
+constexpr uint32_t kCutsExpandTarget = 1u << 25;  // listed tuples before the walks (2^20: 1.13 s, 2^25: 0.36 s)
+constexpr uint32_t kCutsListMax = 1u << 26;       // largest expansion level (16 B per entry)
 constexpr uint32_t kCutsListCap = 1u << 24;       // receiver points per call
 constexpr int kCutsMaxLevels = 64;
 constexpr uint32_t kCutsOnePassCap = 1u << 26;   // records: larger capacities take the two-pass walk
