@@ -352,52 +352,74 @@ def bench_c2(args) -> dict:
     for K, ev, d, n in plain_lists:
         event_share[ev] = event_share.get(ev, 0) + n
     flops = _solver_flops(stats, event_share)
-    dominant = max(kern, key=kern.get)
-    units = {}  # kernel -> (paths, bytes per path, items, bytes per item)
+    # the kernels of the solve as the autotuner structured it (fused mask bit 1:
+    # one Newton kernel over every chain length, bit 2: one occlusion kernel),
+    # each with its compulsory bytes: (paths, bytes per path, items, bytes per
+    # item) per chain length it covers
+    mask = bs.get_fused()[1]
+    waves = bs.get_config(0)["chunk"]
+    kernels = {}  # key -> (kernel name, ms, [(K, paths, bytes/path, items, bytes/item)])
+
+    def newton_units(K):
+        return (K, int(ranges[K]) - int(ranges[K - 1]), NEWTON_BYTES(K)[0], stats[f"K{K}"]["items"],
+                NEWTON_BYTES(K)[1])
+
+    def occlude_units(K):
+        return (K, path_stats[f"K{K}"], OCCLUDE_BYTES(K)[0], stats[f"K{K}"]["converged"], OCCLUDE_BYTES(K)[1])
+    if mask & 1:
+        kernels["newton_mixed"] = (f"k_newton_mixed<{waves}>", kern["newton_k1"], [newton_units(K) for K in (1, 2, 3)])
+    else:
+        for K in (1, 2, 3):
+            kernels[f"newton_k{K}"] = (f"k_newton<{K}>", kern[f"newton_k{K}"], [newton_units(K)])
+    if mask & 2:
+        kernels["occlude_mixed"] = ("k_occlude_mixed", kern["occlude_k1"], [occlude_units(K) for K in (1, 2, 3)])
+    else:
+        for K in (1, 2, 3):
+            kernels[f"occlude_k{K}"] = (f"k_occlude<{K}>", kern[f"occlude_k{K}"], [occlude_units(K)])
     for K in (1, 2, 3):
-        row = stats[f"K{K}"]
-        n_paths_k = int(ranges[K]) - int(ranges[K - 1])
-        units[f"newton_k{K}"] = (n_paths_k, *NEWTON_BYTES(K)[:1], row["items"], NEWTON_BYTES(K)[1])
-        units[f"occlude_k{K}"] = (path_stats[f"K{K}"], OCCLUDE_BYTES(K)[0], row["converged"], OCCLUDE_BYTES(K)[1])
-        units[f"finish_k{K}"] = (path_stats[f"K{K}"], FINISH_BYTES(K)[0], row["converged"], FINISH_BYTES(K)[1])
-    kernel_name = {f"newton_k{K}": f"k_newton<{K}>" for K in (1, 2, 3)}
-    kernel_name.update({f"occlude_k{K}": f"k_occlude<{K}>" for K in (1, 2, 3)})
-    kernel_name.update({f"finish_k{K}": f"k_finish<{K}, false>" for K in (1, 2, 3)})
-    kernel_name.update({"compact": "k_compact_scatter", "refine": "k_refine<K>", "cull": "k_cull_paths"})
-    roof = {"bound": "hbm", "kernel": kernel_name[dominant], "kernel_ms": round(kern[dominant], 3)}
-    if dominant in units:
-        n_p, per_p, n_i, per_i = units[dominant]
-        algo = n_p * per_p + n_i * per_i
-        achieved = algo / (kern[dominant] * 1e-3) / 1e9
+        kernels[f"finish_k{K}"] = (f"k_finish<{K}, 1>", kern[f"finish_k{K}"],
+                                   [(K, path_stats[f"K{K}"], FINISH_BYTES(K)[0], stats[f"K{K}"]["converged"],
+                                     FINISH_BYTES(K)[1])])
+    kernels["compact"] = ("k_compact_scatter", kern["compact"], None)
+    dominant = max(kernels, key=lambda k: kernels[k][1])
+    name, dom_ms, dom_units = kernels[dominant]
+    roof = {"bound": "hbm", "kernel": name, "kernel_ms": round(dom_ms, 3), "fused_mask": mask}
+    if dom_units:
+        algo = sum(n_p * per_p + n_i * per_i for _, n_p, per_p, n_i, per_i in dom_units)
+        achieved = algo / (dom_ms * 1e-3) / 1e9
         roof.update({"achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "algorithmic_bytes": algo,
-                     "bytes_per_path": per_p, "paths": n_p, "bytes_per_item": per_i, "items": n_i})
-    traffic = profiled_traffic("c2", kernel_name[dominant])
-    roof["traffic"] = traffic
+                     "units": {f"K{K}": {"paths": n_p, "bytes_per_path": per_p, "items": n_i, "bytes_per_item": per_i}
+                               for K, n_p, per_p, n_i, per_i in dom_units}})
+    roof["traffic"] = profiled_traffic("c2", name)
     # the occlusion kernels are traversal-bound: rays per second and the
     # escape-link walk's work per ray (oracle counts on the sample)
     ow = sample.get("occlusion_work")
     if ow and ow["converged_items"]:
         rays_per_item = ow["rays"] / ow["converged_items"]
         traversal = {}
-        for K in (1, 2, 3):
-            t_ms = kern.get(f"occlude_k{K}", 0.0)
-            conv_k = stats[f"K{K}"]["converged"]
-            if t_ms > 0 and conv_k:
-                traversal[f"k_occlude<{K}>"] = {
-                    "converged_items_per_s": round(conv_k / (t_ms * 1e-3), 1),
-                    "rays_per_s_est": round(conv_k * rays_per_item / (t_ms * 1e-3), 1)}
+        for key, (kname, t_ms, kunits) in kernels.items():
+            if not key.startswith("occlude"):
+                continue
+            conv = sum(n_i for _, _, _, n_i, _ in kunits)
+            if t_ms > 0 and conv:
+                traversal[kname] = {"converged_items_per_s": round(conv / (t_ms * 1e-3), 1),
+                                    "rays_per_s_est": round(conv * rays_per_item / (t_ms * 1e-3), 1)}
         traversal["sample"] = {"rays_per_converged_item": round(rays_per_item, 3),
                                "boxes_per_ray": round(ow["boxes"] / max(ow["rays"], 1), 2),
                                "triangles_per_ray": round(ow["triangles"] / max(ow["rays"], 1), 2),
                                "walk": "escape-link binary walk (minimum_lbvh.h:1078-1152), oracle counts",
                                "paths": ow["paths"]}
         roof["traversal"] = traversal
-    valu = {f"k_newton<{K}>": {"achieved_TFLOPs": round(flops[K] / (kern[f"newton_k{K}"] * 1e-3) / 1e12, 2),
-                               "peak_TFLOPs": FP32_PEAK_TFLOPS,
-                               "frac": round(flops[K] / (kern[f"newton_k{K}"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
-                               "flops": flops[K]}
-            for K in (1, 2, 3) if kern[f"newton_k{K}"] > 0 and flops[K] > 0}
+    valu = {}
+    for key, (kname, t_ms, kunits) in kernels.items():
+        if not key.startswith("newton"):
+            continue
+        f = sum(flops[K] for K, *_ in kunits)
+        if t_ms > 0 and f > 0:
+            valu[kname] = {"achieved_TFLOPs": round(f / (t_ms * 1e-3) / 1e12, 2), "peak_TFLOPs": FP32_PEAK_TFLOPS,
+                           "frac": round(f / (t_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4), "flops": f,
+                           "counted": "reference-equivalent FLOPs (2K single-tangent passes per iteration)"}
     trace = sample.get("photon_trace_cpu")
     if trace:
         # photonTrace's algorithmic bytes (64 per node visited, 36 per triangle
@@ -584,7 +606,7 @@ def bench_c4(args) -> dict:
             bs.set_config(K, *c)
         if len(cfg) > 3:
             bs.set_config(0, *cfg[3])
-        bs.set_fused(1 if len(cfg) > 3 else 0)
+        bs.set_fused(3 if len(cfg) > 3 else 0)
         tune = [bs.get_config(K) for K in (1, 2, 3)]
     else:
         tune = bs.autotune(batch.d_scenes, len(scene_list), L, refine=CA.REFINE_VERTICES)

@@ -621,15 +621,16 @@ int palz_batched_order(palz_batched* b, const uint32_t** d_order, const palz_pat
  * in waves per SIMD, 2 or 3 (each length keeps its own queue chunk). */
 int palz_batched_set_config(palz_batched* b, int K, const palz_launch_config* cfg);
 int palz_batched_get_config(palz_batched* b, int K, palz_launch_config* cfg);
-/* Fused mode (no counterpart in the reference): 1 = one persistent Newton
- * kernel and one occlusion kernel over every chain length (each wave drains
- * the K = 3 queue, then K = 2's, then K = 1's) in place of one kernel per
- * length; the finish kernels stay per length.  0 = one Newton and one
- * occlusion kernel per chain length.  -1 (default) = per length until
- * palz_batched_autotune has timed whole solves both ways and kept the faster.
- * Results are the same either way.  Drops the captured graph. */
+/* Fused mode (no counterpart in the reference), a bit mask: 1 = one
+ * persistent Newton kernel over every chain length (each wave drains the
+ * K = 3 queue, then K = 2's, then K = 1's) in place of one per length; 2 = one
+ * occlusion kernel over every chain length likewise; 3 = both; 0 = one of
+ * each per chain length.  The finish kernels stay per length.  -1 (default)
+ * = per length until palz_batched_autotune has timed whole solves in all four
+ * structures and kept the fastest.  Results are the same in every structure.
+ * Drops the captured graph. */
 int palz_batched_set_fused(palz_batched* b, int mode);
-/* The fused mode set and whether solves now run fused (0 or 1). */
+/* The fused mode set and the mask solves now run with (0..3). */
 int palz_batched_get_fused(palz_batched* b, int* mode, int* active);
 /* Solve every prepared item.  d_scenes: device array of n_scenes scene
  * descriptors (a path's `scene` indexes it); every scene's lambdas array

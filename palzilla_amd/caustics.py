@@ -381,16 +381,19 @@ class BatchedSolver:
         return out
 
     def set_fused(self, mode: int | bool) -> None:
-        """1/True: one persistent Newton and one occlusion kernel over every
-        chain length; 0/False: one of each per chain length; -1 (the default):
-        chosen by the autotuner (per chain length until it has run)."""
-        call("palz_batched_set_fused", C.c_void_p(self.handle), int(mode))
+        """A bit mask: 1 = one persistent Newton kernel over every chain
+        length, 2 = one occlusion kernel over every chain length, 3 (or True)
+        = both, 0 (or False) = one of each per chain length; -1 (the
+        default): chosen by the autotuner (per chain length until it has
+        run)."""
+        mode = 3 if mode is True else int(mode)
+        call("palz_batched_set_fused", C.c_void_p(self.handle), mode)
 
-    def get_fused(self) -> tuple[int, bool]:
-        """(mode set, whether solves now run fused)."""
+    def get_fused(self) -> tuple[int, int]:
+        """(mode set, the mask solves now run with)."""
         mode, active = C.c_int(), C.c_int()
         call("palz_batched_get_fused", C.c_void_p(self.handle), C.byref(mode), C.byref(active))
-        return mode.value, bool(active.value)
+        return mode.value, active.value
 
     def set_tune_cache(self, path: str | None) -> None:
         """The autotuner's tile-shape cache file (None: no cache)."""

@@ -709,12 +709,13 @@ struct palz_batched {
     // kernel's register budget in waves per SIMD (2 or 3), occlusion block);
     // each length keeps its own queue chunk from cfg
     palz_launch_config cfg_mixed = {256, 2, 2, 256};
-    // fused mode: 1 = one persistent Newton and one occlusion kernel over every
-    // chain length, 0 = one of each per length, -1 = chosen by the autotuner
-    // (per length until it has run)
+    // fused mode, a bit mask: 1 = one persistent Newton kernel over every chain
+    // length, 2 = one occlusion kernel over every chain length (3 = both, 0 =
+    // one of each per length); -1 = chosen by the autotuner (per length until
+    // it has run)
     int fused = -1;
-    bool auto_fused = false;  // the autotuner's choice in mode -1
-    bool fused_on() const { return fused == 1 || (fused == -1 && auto_fused); }
+    int auto_fused = 0;  // the autotuner's choice in mode -1
+    int fused_mask() const { return fused == -1 ? auto_fused : fused; }
     std::string tune_cache;  // the autotuner's tile-shape cache file ("" = none)
     int cus = 256;
     // hipGraph cache
@@ -877,45 +878,32 @@ struct palz_batched {
         if (t) PALZ_HIP(hipEventRecord(ev[0], st));
         if (cull) launch_cull_paths(scenes, n_scenes, paths, cap_paths, ranges + 5, admissible, st, cull);
         if (t) PALZ_HIP(hipEventRecord(ev[9], st));
-        if (fused_on()) {
-            // the fused kernels' times go to the K = 1 slots; K = 2 and 3 stay ~0
+        const int f = fused_mask();
+        // a fused kernel's time goes to its K = 1 slot; the K = 2 and 3 slots stay ~0
+        if (f & 1) {
             newton_mixed(cfg_mixed, scenes, n_scenes, L, results, st);
             for (int i : {1, 2, 3})
                 if (t) PALZ_HIP(hipEventRecord(ev[i], st));
-            compact(L, results, st);
-            if (t) PALZ_HIP(hipEventRecord(ev[4], st));
-            occlude_mixed(cfg_mixed, scenes, L, st);
-            if (t) PALZ_HIP(hipEventRecord(ev[10], st));
-            finish<1>(cfg[0], scenes, L, refine, results, refined, st);
-            if (t) PALZ_HIP(hipEventRecord(ev[5], st));
-            if (t) PALZ_HIP(hipEventRecord(ev[11], st));
-            finish<2>(cfg[1], scenes, L, refine, results, refined, st);
-            if (t) PALZ_HIP(hipEventRecord(ev[6], st));
-            if (t) PALZ_HIP(hipEventRecord(ev[12], st));
-            finish<3>(cfg[2], scenes, L, refine, results, refined, st);
-            if (t) PALZ_HIP(hipEventRecord(ev[7], st));
-            if (t) PALZ_HIP(hipEventRecord(ev[8], st));
-            PALZ_LAUNCH_CHECK();
-            if (t) timed_solves = true;
-            return PALZ_OK;
+        } else {
+            newton<1>(cfg[0], scenes, n_scenes, L, results, st);
+            if (t) PALZ_HIP(hipEventRecord(ev[1], st));
+            newton<2>(cfg[1], scenes, n_scenes, L, results, st);
+            if (t) PALZ_HIP(hipEventRecord(ev[2], st));
+            newton<3>(cfg[2], scenes, n_scenes, L, results, st);
+            if (t) PALZ_HIP(hipEventRecord(ev[3], st));
         }
-        newton<1>(cfg[0], scenes, n_scenes, L, results, st);
-        if (t) PALZ_HIP(hipEventRecord(ev[1], st));
-        newton<2>(cfg[1], scenes, n_scenes, L, results, st);
-        if (t) PALZ_HIP(hipEventRecord(ev[2], st));
-        newton<3>(cfg[2], scenes, n_scenes, L, results, st);
-        if (t) PALZ_HIP(hipEventRecord(ev[3], st));
         compact(L, results, st);
         if (t) PALZ_HIP(hipEventRecord(ev[4], st));
-        occlude<1>(cfg[0], scenes, L, st);
+        if (f & 2) occlude_mixed(cfg_mixed, scenes, L, st);
+        if (!(f & 2)) occlude<1>(cfg[0], scenes, L, st);
         if (t) PALZ_HIP(hipEventRecord(ev[10], st));
         finish<1>(cfg[0], scenes, L, refine, results, refined, st);
         if (t) PALZ_HIP(hipEventRecord(ev[5], st));
-        occlude<2>(cfg[1], scenes, L, st);
+        if (!(f & 2)) occlude<2>(cfg[1], scenes, L, st);
         if (t) PALZ_HIP(hipEventRecord(ev[11], st));
         finish<2>(cfg[1], scenes, L, refine, results, refined, st);
         if (t) PALZ_HIP(hipEventRecord(ev[6], st));
-        occlude<3>(cfg[2], scenes, L, st);
+        if (!(f & 2)) occlude<3>(cfg[2], scenes, L, st);
         if (t) PALZ_HIP(hipEventRecord(ev[12], st));
         finish<3>(cfg[2], scenes, L, refine, results, refined, st);
         if (t) PALZ_HIP(hipEventRecord(ev[7], st));
@@ -938,7 +926,7 @@ bool valid_config(const palz_launch_config& c, bool mixed = false) {
 // count, refinement, culling and fused modes and, per chain length, its item
 // count rounded up to a power of two; workloads with equal keys take the same
 // launch shapes.  The file holds one line per tuning,
-//     palz-tune <10 key fields> : <cfg K=1,2,3 and mixed: 4 x 4 fields> <6 times in us> <fused chosen>
+//     palz-tune <10 key fields> : <cfg K=1,2,3 and mixed: 4 x 4 fields> <6 times in us> <fused mask chosen>
 // and the last line with a matching key wins.
 constexpr int kTuneKeyFields = 10;
 constexpr int kTuneFields = 4 * 4 + 6 + 1;
@@ -952,7 +940,7 @@ int ceil_log2_plus1(uint64_t x) {
 }
 TuneKey tune_key(int cus, const uint32_t* ranges, uint32_t n_scenes, uint32_t L, int refine, int cull, int fused) {
     TuneKey k;
-    k.v[0] = 2;  // format version
+    k.v[0] = 3;  // format version
     k.v[1] = cus;
     k.v[2] = ceil_log2_plus1(n_scenes);
     k.v[3] = (int)L;
@@ -1108,8 +1096,8 @@ int palz_batched_get_config(palz_batched* b, int K, palz_launch_config* cfg) {
 }
 
 int palz_batched_set_fused(palz_batched* b, int mode) {
-    if (!b || mode < -1 || mode > 1)
-        return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_set_fused: mode must be -1, 0 or 1");
+    if (!b || mode < -1 || mode > 3)
+        return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_set_fused: mode must be -1, 0, 1, 2 or 3");
     b->fused = mode;
     b->drop_graph();
     return PALZ_OK;
@@ -1118,7 +1106,7 @@ int palz_batched_set_fused(palz_batched* b, int mode) {
 int palz_batched_get_fused(palz_batched* b, int* mode, int* active) {
     if (!b) return palz::fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_get_fused: NULL batch");
     if (mode) *mode = b->fused;
-    if (active) *active = b->fused_on() ? 1 : 0;
+    if (active) *active = b->fused_mask();
     return PALZ_OK;
 }
 
@@ -1135,7 +1123,7 @@ int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, u
     int rc = check_solve_args(b, d_scenes, n_scenes, n_lambdas, refine);
     if (rc != PALZ_OK) return rc;
     hipStream_t st = (hipStream_t)stream;
-    palz_batched::Key k{d_scenes, d_results, d_refined, n_scenes, n_lambdas, refine, {}, b->cull, b->fused_on(),
+    palz_batched::Key k{d_scenes, d_results, d_refined, n_scenes, n_lambdas, refine, {}, b->cull, b->fused_mask(),
                         b->cfg_mixed};
     std::copy(b->cfg, b->cfg + 3, k.cfg);
     const bool same = b->exec && std::memcmp(&k, &b->key, sizeof(k)) == 0;
@@ -1144,8 +1132,8 @@ int palz_batched_solve_graph(palz_batched* b, const palz_scene_desc* d_scenes, u
         // Branches pay when no chain length dominates (many scenes of mixed
         // lengths); when one length holds most paths its kernels fill the GPU
         // and the others only compete with them.
-        bool branch = b->branches == 1 && !b->fused_on();
-        if (b->branches < 0 && !b->fused_on()) {
+        bool branch = b->branches == 1 && b->fused_mask() == 0;
+        if (b->branches < 0 && b->fused_mask() == 0) {
             uint32_t r[6];
             PALZ_HIP(hipMemcpy(r, b->ranges, sizeof(r), hipMemcpyDeviceToHost));
             const uint32_t most = std::max(r[1] - r[0], std::max(r[2] - r[1], r[3] - r[2]));
@@ -1264,11 +1252,11 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
             palz_launch_config c[4];
             for (int i = 0; i < 4; ++i) c[i] = {v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]};
             const bool ok = valid_config(c[0]) && valid_config(c[1]) && valid_config(c[2]) &&
-                            valid_config(c[3], true) && (v[22] == 0 || v[22] == 1);
+                            valid_config(c[3], true) && v[22] >= 0 && v[22] <= 3;
             if (ok) {
                 std::copy(c, c + 3, b->cfg);
                 b->cfg_mixed = c[3];
-                b->auto_fused = v[22] == 1;
+                b->auto_fused = v[22];
                 for (int i = 0; i < 6; ++i) times_us[i] = v[16 + i];
                 write_report();
                 b->drop_graph();
@@ -1325,7 +1313,7 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
     }
     // the mixed-length Newton kernel: register budget (waves per SIMD), block
     // size and resident blocks per CU, with each length's tuned queue chunk
-    if (b->fused != 0) {
+    if (b->fused == -1 || (b->fused & 1)) {
         float best_ms = 1e30f;
         palz_launch_config best = b->cfg_mixed;
         for (int waves : {2, 3})
@@ -1374,7 +1362,7 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
         finish_ms[K - 1] = best_fin;
     }
     // the mixed-length occlusion kernel's block size
-    if (b->fused != 0) {
+    if (b->fused == -1 || (b->fused & 2)) {
         float best_ms = 1e30f;
         for (int fb : blocks) {
             palz_launch_config c = b->cfg_mixed;
@@ -1388,18 +1376,22 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
             }
         }
     }
-    // mode -1: the whole solve both ways on the tuned shapes, the faster kept
+    // mode -1: the whole solve in each of the four structures (Newton and
+    // occlusion each per length or fused) on the tuned shapes, the fastest kept
     if (b->fused == -1) {
-        float best[2] = {1e30f, 1e30f};
+        float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
         for (int rep = 0; rep < 2; ++rep)
-            for (int f = 0; f < 2; ++f) {
-                b->auto_fused = f == 1;
+            for (int f = 0; f < 4; ++f) {
+                b->auto_fused = f;
                 (void)hipEventRecord(e0, st);
                 rc = b->launch_all(d_scenes, n_scenes, n_lambdas, refine, nullptr, nullptr, st, false);
                 if (rc != PALZ_OK) return rc;
                 best[f] = std::min(best[f], elapsed());
             }
-        b->auto_fused = best[1] < best[0];
+        int pick = 0;
+        for (int f = 1; f < 4; ++f)
+            if (best[f] < best[pick]) pick = f;
+        b->auto_fused = pick;
     }
     for (int K = 0; K < 3; ++K) {
         times_us[2 * K] = (int32_t)(newton_ms[K] * 1000.0f);
@@ -1420,7 +1412,7 @@ int palz_batched_autotune(palz_batched* b, const palz_scene_desc* d_scenes, uint
             v[4 * i + 3] = c4[i]->finish_block;
         }
         for (int i = 0; i < 6; ++i) v[16 + i] = times_us[i];
-        v[22] = b->auto_fused ? 1 : 0;
+        v[22] = b->auto_fused;
         if (!tune_cache_append(b->tune_cache, key, v))
             return fail(PALZ_ERR_INVALID_ARGUMENT, "palz_batched_autotune: cannot append to the tile-shape cache file");
     }

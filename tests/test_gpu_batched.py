@@ -313,7 +313,7 @@ def test_autotune_keeps_results(run):
     # mode -1: the autotuner timed whole solves per length and fused and kept one
     mode, active = run.bs.get_fused()
     assert mode == -1
-    run.bs.set_fused(0 if active else 1)
+    run.bs.set_fused((active + 1) % 4)
     other_mode, _, _ = run.solve(refine=CA.REFINE_VERTICES)
     run.bs.set_fused(-1)
     assert run.bs.get_fused() == (-1, active)
@@ -328,16 +328,18 @@ def test_fused_kernels_match_per_length_kernels(run):
     # the same items, results and refined vertices as one kernel per length,
     # eagerly and in graph replays
     split, split_ref, split_img = run.solve(refine=CA.REFINE_VERTICES)
-    run.bs.set_fused(True)
     try:
-        # the fused Newton kernel's register budget: 2 or 3 waves per SIMD
-        for waves in (2, 3):
+        # fused Newton (1), fused occlusion (2), both (3); the fused Newton
+        # kernel's register budget: 2 or 3 waves per SIMD
+        for mask, waves in ((1, 2), (2, 2), (3, 2), (3, 3)):
+            run.bs.set_fused(mask)
             run.bs.set_config(0, 256, 2, waves, 256)
             for graph in (False, True, True):
                 res, ref, img = run.solve(refine=CA.REFINE_VERTICES, graph=graph)
-                assert res.tobytes() == split.tobytes(), f"waves={waves} graph={graph}"
-                assert ref.tobytes() == split_ref.tobytes(), f"waves={waves} graph={graph}"
-                assert np.allclose(img, split_img, rtol=1e-5, atol=1e-6), f"waves={waves} graph={graph}"
+                case = f"mask={mask} waves={waves} graph={graph}"
+                assert res.tobytes() == split.tobytes(), case
+                assert ref.tobytes() == split_ref.tobytes(), case
+                assert np.allclose(img, split_img, rtol=1e-5, atol=1e-6), case
         st = run.bs.stats()
         K = run.paths["event"] >> 24
         for k in (1, 2, 3):
