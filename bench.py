@@ -202,8 +202,12 @@ def bench_c2(args) -> dict:
     d_res = DeviceBuffer(max(int(ranges[5]) * L, 1) * CA.PATH_RESULT.itemsize)
 
     if args.launch_config:
-        for K, c in enumerate(json.loads(args.launch_config)[:3], start=1):
+        cfg = json.loads(args.launch_config)
+        for K, c in enumerate(cfg[:3], start=1):
             bs.set_config(K, *c)
+        if len(cfg) > 3:
+            bs.set_config(0, *cfg[3])
+        bs.set_fused(args.fused)
         tune = [bs.get_config(K) for K in (1, 2, 3)]
         log(f"[bench] c2: launch shapes {tune}")
     else:
@@ -606,7 +610,7 @@ def bench_c4(args) -> dict:
             bs.set_config(K, *c)
         if len(cfg) > 3:
             bs.set_config(0, *cfg[3])
-        bs.set_fused(3 if len(cfg) > 3 else 0)
+        bs.set_fused(args.fused if args.fused >= 0 else (3 if len(cfg) > 3 else 0))
         tune = [bs.get_config(K) for K in (1, 2, 3)]
     else:
         tune = bs.autotune(batch.d_scenes, len(scene_list), L, refine=CA.REFINE_VERTICES)
@@ -966,7 +970,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5", "lbvh", "pathcuts"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5", "lbvh", "pathcuts", "renderer"])
     ap.add_argument("--no-extras", action="store_true", help="c2 only, without the c4 and lbvh sub-results")
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--cpu-sample", type=int, default=1 << 22, help="LBVH CPU baseline triangles")
@@ -977,6 +981,9 @@ def main() -> int:
                     help="skip the c2 autotuner: JSON [[block, blocks_per_cu, chunk, finish_block]] for K = 1..3 "
                          "and, optionally, the fused kernels' [block, blocks_per_cu, waves per SIMD, occlusion block]")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fused", type=int, default=-1,
+                    help="with --launch-config: the fused mask (1 Newton, 2 occlusion, 3 both; -1: per length "
+                         "for c2, as the shapes say for c4)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); nothing is run across GPUs.",
@@ -1000,6 +1007,10 @@ def main() -> int:
         out = {"metric": METRIC, "value": round(r["params"] / (r["backward_ms"] * 1e-3), 1),
                "unit": "height gradients/s", "ms_per_step": r["forward_ms"] + r["backward_ms"], "dtype": "f32+f64",
                "config": {"workload": r["workload"]}, "c5": r}
+    elif args.workload == "renderer":
+        r = bench_renderer(args)
+        out = {"metric": METRIC, "value": r["frames_per_s"], "unit": "frames/s", "ms_per_step": r["ms_per_frame"],
+               "dtype": "f32+f64", "config": {"workload": r["workload"]}, "renderer": r}
     elif args.workload == "c4":
         r = bench_c4(args)
         out = {"metric": METRIC, "value": r["batched_paths_per_s"], "unit": "paths/s",

@@ -352,6 +352,7 @@ namespace {
 // Items per workgroup (one wave): their FP64 solutions and pseudo-inverses
 // stay in LDS between the two phases of k_adjoint.
 constexpr int kAdjItems = 64;
+constexpr uint32_t kAdjSpread = 4096;  // waves per window (see k_adjoint)
 
 // An item's FP64 solution x* and the pseudo-inverse J_x^+ (P x R, row-major)
 // of its constraint Jacobian there.
@@ -525,7 +526,7 @@ __device__ double adjoint_input(const palz_scene_desc& sc, const palz_path& path
 }
 
 // d(loss)/d(heights) of the contributing items of chain length K.  One wave
-// per kAdjItems consecutive items: phase A prepares each item in its own lane
+// per kAdjItems items: phase A prepares each item in its own lane
 // (x*, J_x^+ into LDS); phase B gives every (item, parameter vertex, input)
 // triple a lane of its own, four lanes per vertex: the height and the three
 // shading-normal components, each a single-tangent forward pass.  The four
@@ -541,11 +542,17 @@ __global__ void __launch_bounds__(kAdjItems)
     const uint32_t L = sc.n_lambdas;
     const uint32_t lane = threadIdx.x;
     const uint64_t n_items = (uint64_t)n_paths * L;
-    const uint64_t base = (uint64_t)blockIdx.x * kAdjItems;
+    // Windows of kAdjSpread waves: lane l of the window's wave j takes item
+    // l * kAdjSpread + j.  Callers pass lists ordered by region (the batched
+    // solver's), whose neighbouring items add into the same heights; spread
+    // kAdjSpread apart, a wave's FP64 atomics seldom meet on one address,
+    // while a window stays a contiguous part of the list (one chain length).
+    const uint64_t window = (uint64_t)(blockIdx.x / kAdjSpread) * (kAdjItems * kAdjSpread);
+    const uint32_t j = blockIdx.x % kAdjSpread;
     // phase A
     uint32_t mask = 0u;
     {
-        const uint64_t item = base + lane;
+        const uint64_t item = window + (uint64_t)lane * kAdjSpread + j;
         if (item < n_items) {
             const uint32_t pi = (uint32_t)(item / L), l = (uint32_t)(item % L);
             const palz_path path = paths[pi];
@@ -576,7 +583,7 @@ __global__ void __launch_bounds__(kAdjItems)
         int h = -1;
         palz_path path;
         if (active) {
-            const uint64_t item = base + il;
+            const uint64_t item = window + (uint64_t)il * kAdjSpread + j;
             const uint32_t pi = (uint32_t)(item / L), l = (uint32_t)(item % L);
             path = paths[pi];
             const float w_img = weight[(size_t)l * sc.n_points + path.pixel];
@@ -627,7 +634,8 @@ int palz_adjoint_irradiance(const palz_scene_desc* scene, const palz_height_para
     const uint64_t items = (uint64_t)n_paths * scene->n_lambdas;
     if (items == 0) return PALZ_OK;
     if (items >= (1ull << 31)) return fail(PALZ_ERR_CAPACITY, "palz_adjoint_irradiance: more than 2^31 items");
-    const uint32_t blocks = (uint32_t)((items + kAdjItems - 1) / kAdjItems);
+    const uint64_t per_window = (uint64_t)kAdjItems * kAdjSpread;
+    const uint32_t blocks = (uint32_t)((items + per_window - 1) / per_window * kAdjSpread);
     hipStream_t st = (hipStream_t)stream;
     k_adjoint<1><<<blocks, kAdjItems, 0, st>>>(*scene, *hp, d_paths, n_paths, d_results, d_weight, d_grad);
     k_adjoint<2><<<blocks, kAdjItems, 0, st>>>(*scene, *hp, d_paths, n_paths, d_results, d_weight, d_grad);

@@ -69,24 +69,33 @@ __global__ void k_reset_bounds(uint32_t* ranges) {
     }
 }
 
-// The largest scene index and first-triangle index over the paths (ranges[6], [7]).
+// The largest scene index and first-triangle index over the paths (ranges[6],
+// [7]): a grid-stride loop reduced per block, one atomic per block and word
+// (atomics from every wave on the same two words serialise at the L2).
 __global__ void __launch_bounds__(256) k_key_bounds(const palz_path* __restrict__ paths, uint32_t n,
                                                     const uint32_t* __restrict__ d_count, uint32_t* __restrict__ ranges) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t s_max[2][256 / 64];
     const uint32_t m = d_count ? min(n, *d_count) : n;
     uint32_t s = 0u, t = 0u;
-    if (i < m) {
-        s = paths[i].scene;
-        t = (uint32_t)paths[i].tris[0];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        s = max(s, paths[i].scene);
+        t = max(t, (uint32_t)paths[i].tris[0]);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         s = max(s, (uint32_t)__shfl_xor((int)s, o, 64));
         t = max(t, (uint32_t)__shfl_xor((int)t, o, 64));
     }
+    const uint32_t wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(ranges + 6, s);
-        atomicMax(ranges + 7, t);
+        s_max[0][wave] = s;
+        s_max[1][wave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t v = 0u;
+        for (uint32_t w = 0; w < blockDim.x / 64u; ++w) v = max(v, s_max[threadIdx.x][w]);
+        atomicMax(ranges + 6 + threadIdx.x, v);
     }
 }
 
@@ -1054,7 +1063,7 @@ int palz_batched_prepare(palz_batched* b, const palz_path* d_paths, uint32_t n_p
         return PALZ_OK;
     }
     k_reset_bounds<<<1, 64, 0, st>>>(b->ranges);
-    k_key_bounds<<<nb, 256, 0, st>>>(d_paths, n_paths, d_count, b->ranges);
+    k_key_bounds<<<std::min(nb, 1024u), 256, 0, st>>>(d_paths, n_paths, d_count, b->ranges);
     k_size_words<<<nb, 256, 0, st>>>(d_paths, n_paths, d_count, b->ranges, b->words);
     const SortScratch& s = sorter->scratch;
     launch_sort_prepare(s, nullptr, st);
