@@ -236,6 +236,7 @@ def bench_c2(args) -> dict:
     ms = t0.elapsed_ms(t1) / args.steps
     stats = bs.stats()
     path_stats = bs.path_stats()
+    timed_mask = bs.get_fused()[1]  # the structure the timed steps ran
 
     # the same solve with the per-tuple culling in front of Newton: the
     # interval test (mode 1), then the interval test followed by the affine
@@ -360,7 +361,7 @@ def bench_c2(args) -> dict:
     # one Newton kernel over every chain length, bit 2: one occlusion kernel),
     # each with its compulsory bytes: (paths, bytes per path, items, bytes per
     # item) per chain length it covers
-    mask = bs.get_fused()[1]
+    mask = timed_mask
     waves = bs.get_config(0)["chunk"]
     kernels = {}  # key -> (kernel name, ms, [(K, paths, bytes/path, items, bytes/item)])
 
@@ -460,7 +461,7 @@ def bench_c2(args) -> dict:
                     "converged_items": converged, "contributing_items": contributing,
                     "stats": stats, "launch_configs": tune, "wall_s": round(wall, 3),
                     "newton_fp32_roofline": valu, "fused": fused,
-                    "fused_chosen_by_autotuner": bs.get_fused()[1]},
+                    "fused_mask": timed_mask},
         "plain": {"value": round(plain_value, 1), "unit": "paths/s", "ms_per_step": round(ms_plain, 3),
                   "launches": len(plain_lists), "kernel_ms": {k: round(v, 3) for k, v in plain_kernel_ms.items()},
                   "batched_over_plain": round(value / plain_value, 2) if plain_value else None,
@@ -673,6 +674,7 @@ def bench_fused(args, bs, d_scenes, n_scenes, L, d_res, st, graph: bool, check: 
     from palzilla_amd import caustics as CA
     from palzilla_amd.device import DeviceBuffer, Event
     ref = CA.download_results(d_res, int(bs.ranges()[5]) * L) if check else None
+    prev_mode = bs.get_fused()[0]
     bs.set_fused(True)
     try:
         cfg = json.loads(args.launch_config) if args.launch_config else []
@@ -700,7 +702,7 @@ def bench_fused(args, bs, d_scenes, n_scenes, L, d_res, st, graph: bool, check: 
         if ref is not None:
             same = CA.download_results(d_out, ref.shape[0]).tobytes() == ref.tobytes()
     finally:
-        bs.set_fused(-1)
+        bs.set_fused(prev_mode)
     return {"ms_per_step": round(ms, 3), "graph": graph, "results_equal_per_length_kernels": same,
             "newton_mixed_ms": round(k["newton_k1"], 3), "occlude_mixed_ms": round(k["occlude_k1"], 3),
             "finish_ms": {f"K{K}": round(k[f"finish_k{K}"], 3) for K in (1, 2, 3)},
