@@ -77,23 +77,57 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const uint32_t* in,
     const uint32_t m = count_of(n, n_dev);
     const uint32_t base = blockIdx.x * kScanTile;
     if (base >= m) return;
-    // each thread owns kScanItems consecutive items
+    // each thread owns kScanItems consecutive items; a whole tile of 16-byte
+    // aligned buffers moves as uint4s (a wave's loads then cover whole lines
+    // instead of one word per line and instruction)
     uint32_t v[kScanItems];
     const uint32_t t0 = base + threadIdx.x * kScanItems;
+    const bool vec = base + kScanTile <= m && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0u;
     uint32_t acc = 0;
+    if (vec) {
+        const uint4* in4 = reinterpret_cast<const uint4*>(in + t0);
 #pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
-        const uint32_t i = t0 + j;
-        v[j] = i < m ? in[i] : 0u;
-        acc += v[j];
+        for (int q = 0; q < kScanItems / 4; ++q) {
+            const uint4 x = in4[q];
+            v[4 * q + 0] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            v[4 * q + 3] = x.w;
+        }
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) acc += v[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const uint32_t i = t0 + j;
+            v[j] = i < m ? in[i] : 0u;
+            acc += v[j];
+        }
     }
     uint32_t tot;
     uint32_t run = block_excl(acc, s_w, &tot) + sums[blockIdx.x];
+    if (vec) {
+        uint4* out4 = reinterpret_cast<uint4*>(out + t0);
 #pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
-        const uint32_t i = t0 + j;
-        if (i < m) out[i] = run;
-        run += v[j];
+        for (int q = 0; q < kScanItems / 4; ++q) {
+            uint4 y;
+            y.x = run;
+            run += v[4 * q + 0];
+            y.y = run;
+            run += v[4 * q + 1];
+            y.z = run;
+            run += v[4 * q + 2];
+            y.w = run;
+            run += v[4 * q + 3];
+            out4[q] = y;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const uint32_t i = t0 + j;
+            if (i < m) out[i] = run;
+            run += v[j];
+        }
     }
 }
 

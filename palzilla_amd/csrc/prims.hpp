@@ -8,6 +8,7 @@ namespace palz {
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;
+static_assert(kScanItems % 4 == 0, "the apply pass moves full tiles as uint4s");
 
 // Exclusive scan of n u32 values (reduce-then-scan, three launches).  `n_dev`
 // (optional) overrides n with a device-side count, in which case n is the
