@@ -432,15 +432,26 @@ PALZ_HD v3 inv_rd(v3 rd) {
     return mk(hmax(-kFltMax, hmin(q.x, kFltMax)), hmax(-kFltMax, hmin(q.y, kFltMax)),
               hmax(-kFltMax, hmin(q.z, kFltMax)));
 }
+// The slab test (minimum_lbvh.h:82-96).  The reference folds with a<b?a:b;
+// here the folds are fminf/fmaxf, one v_min/v_max instruction each instead
+// of a compare and a select (the traversals issue VALU instructions at the
+// device's ceiling).  The two differ only for NaN operands and in the sign
+// of a zero result.  No operand here is NaN: ro, lo and hi are finite and
+// ird is clamped to +-FLT_MAX, so every t is finite or an infinity.  A zero's
+// sign changes no later comparison: the max/min folds, the scaling by
+// 1.00000024f and the caller's a <= b treat -0 and +0 alike.  So every box
+// decision, and the order of children by entry distance, is the reference's.
+PALZ_HD float fold_min(float a, float b) { return fminf(a, b); }
+PALZ_HD float fold_max(float a, float b) { return fmaxf(a, b); }
 PALZ_HD void slabs(v3 ro, v3 ird, v3 lo, v3 hi, float known_t, float* rmin, float* rmax) {
     const v3 t0 = (lo - ro) * ird;
     const v3 t1 = (hi - ro) * ird;
-    const v3 tmin = vmin(t0, t1);
-    const v3 tmax = vmax(t0, t1);
-    float a = hmax(hmax(tmin.x, tmin.y), tmin.z);
-    float b = hmin(hmin(tmax.x, tmax.y), tmax.z) * 1.00000024f;
-    a = hmax(a, 0.0f);
-    b = hmin(b, known_t);
+    const v3 tmin = {fold_min(t0.x, t1.x), fold_min(t0.y, t1.y), fold_min(t0.z, t1.z)};
+    const v3 tmax = {fold_max(t0.x, t1.x), fold_max(t0.y, t1.y), fold_max(t0.z, t1.z)};
+    float a = fold_max(fold_max(tmin.x, tmin.y), tmin.z);
+    float b = fold_min(fold_min(tmax.x, tmax.y), tmax.z) * 1.00000024f;
+    a = fold_max(a, 0.0f);
+    b = fold_min(b, known_t);
     *rmin = a;
     *rmax = b;
 }
