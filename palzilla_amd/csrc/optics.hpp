@@ -762,6 +762,32 @@ struct PacketStack {
     uint32_t node[kPacketDepth];
 };
 
+// A lane's membership in a wave-uniform lane mask as a branch condition: the
+// mask becomes the exec mask, with no vector instruction.
+__device__ __forceinline__ bool lane_in(uint64_t lanes) { return __builtin_amdgcn_inverse_ballot_w64(lanes); }
+// The first active lane's bit (the lane readfirstlane reads).
+__device__ __forceinline__ uint64_t first_lane() {
+    const uint64_t active = __ballot(1);
+    return active & (~active + 1ull);
+}
+// A value every lane holds alike, read back as wave-uniform (an LDS load's
+// result is otherwise taken as per-lane, and so is all that is computed from
+// it).
+__device__ __forceinline__ uint32_t uniform32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    return (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32 |
+           __builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
+// The box test of a packet walk: every lane runs it (lanes outside the walk's
+// mask hold rays that are never read back), and the caller keeps the ballot's
+// bits of the lanes it asks about.
+__device__ __forceinline__ bool box_test(v3 ro, v3 ird, const palz_aabb& b, float t_max) {
+    float a, c;
+    slabs(ro, ird, mk(b.lower), mk(b.upper), t_max, &a, &c);
+    return a <= c;
+}
+
 __device__ inline bool any_hit_packet(const palz_internal_node* __restrict__ nodes,
                                       const palz_triangle* __restrict__ tris, uint32_t root, bool query, v3 ro,
                                       v3 rd, v3 ird, float t_max, PacketStack* stack) {
@@ -771,7 +797,7 @@ __device__ inline bool any_hit_packet(const palz_internal_node* __restrict__ nod
     if (pending == 0ull) return false;
     auto test_leaf = [&](uint32_t leaf, uint64_t lanes) {
         const palz_triangle tr = load_uniform(tris + (__builtin_amdgcn_readfirstlane(leaf) & ~PALZ_NODE_LEAF_BIT));
-        if (lanes & me) {
+        if (lane_in(lanes)) {
             float t, u, v;
             v3 ng;
             if (intersect_tri(&t, &u, &v, &ng, 0.0f, t_max, ro, rd, mk(tr.vs[0]), mk(tr.vs[1]), mk(tr.vs[2])))
@@ -790,16 +816,9 @@ __device__ inline bool any_hit_packet(const palz_internal_node* __restrict__ nod
     for (;;) {
         const palz_internal_node nd = load_uniform(nodes + __builtin_amdgcn_readfirstlane(node));
         const uint32_t c0 = nd.children[0], c1 = nd.children[1];
-        const palz_aabb b0 = nd.aabbs[0], b1 = nd.aabbs[1];
-        bool h0 = false, h1 = false;
-        if (mask & me) {
-            float a, b;
-            slabs(ro, ird, mk(b0.lower), mk(b0.upper), t_max, &a, &b);
-            h0 = a <= b;
-            slabs(ro, ird, mk(b1.lower), mk(b1.upper), t_max, &a, &b);
-            h1 = a <= b;
-        }
-        uint64_t m0 = __ballot(h0), m1 = __ballot(h1);
+        // every lane tests the boxes; the ballots keep the lanes of `mask`
+        uint64_t m0 = __ballot(box_test(ro, ird, nd.aabbs[0], t_max)) & mask;
+        uint64_t m1 = __ballot(box_test(ro, ird, nd.aabbs[1], t_max)) & mask;
         if (m0 != 0ull && (c0 & PALZ_NODE_LEAF_BIT)) {
             test_leaf(c0, m0);
             m0 = 0ull;
@@ -815,7 +834,7 @@ __device__ inline bool any_hit_packet(const palz_internal_node* __restrict__ nod
                 overflow = true;
                 break;
             }
-            if ((threadIdx.x & 63) == __builtin_amdgcn_readfirstlane(threadIdx.x & 63)) {
+            if (lane_in(first_lane())) {
                 stack->mask[sp] = m1;
                 stack->node[sp] = c1;
             }
@@ -833,8 +852,8 @@ __device__ inline bool any_hit_packet(const palz_internal_node* __restrict__ nod
             while (sp > 0 && mask == 0ull) {
                 --sp;
                 __builtin_amdgcn_wave_barrier();
-                node = stack->node[sp];
-                mask = stack->mask[sp] & pending;
+                node = uniform32(stack->node[sp]);
+                mask = uniform64(stack->mask[sp]) & pending;
             }
             if (mask == 0ull) break;
         }
@@ -859,7 +878,7 @@ __device__ inline bool any_hit_wide(const palz_internal_node* __restrict__ nodes
     if (pending == 0ull) return false;
     auto test_leaf = [&](uint32_t leaf, uint64_t lanes) {
         const palz_triangle tr = load_uniform(tris + (__builtin_amdgcn_readfirstlane(leaf) & ~PALZ_NODE_LEAF_BIT));
-        if (lanes & me) {
+        if (lane_in(lanes)) {
             float t, u, v;
             v3 ng;
             if (intersect_tri(&t, &u, &v, &ng, 0.0f, t_max, ro, rd, mk(tr.vs[0]), mk(tr.vs[1]), mk(tr.vs[2])))
@@ -867,19 +886,14 @@ __device__ inline bool any_hit_wide(const palz_internal_node* __restrict__ nodes
         }
         pending &= ~__ballot(hit);
     };
-    auto box_hit = [&](const palz_aabb& b, uint64_t lanes) -> uint64_t {
-        bool h = false;
-        if (lanes & me) {
-            float a, c;
-            slabs(ro, ird, mk(b.lower), mk(b.upper), t_max, &a, &c);
-            h = a <= c;
-        }
-        return __ballot(h);
-    };
     if (root & PALZ_NODE_LEAF_BIT) {
         test_leaf(root, pending);
         return hit;
     }
+    // every lane tests the box; the ballot keeps the lanes of `lanes`
+    auto box_hit = [&](const palz_aabb& b, uint64_t lanes) -> uint64_t {
+        return __ballot(box_test(ro, ird, b, t_max)) & lanes;
+    };
     int sp = 0;
     bool overflow = false;
     uint32_t node = root;
@@ -890,7 +904,6 @@ __device__ inline bool any_hit_wide(const palz_internal_node* __restrict__ nodes
         // the first internal grandchild hit is walked next, the others pushed
         uint32_t first = PALZ_NODE_INVALID;
         uint64_t first_mask = 0ull;
-        const bool leader = (threadIdx.x & 63) == __builtin_amdgcn_readfirstlane(threadIdx.x & 63);
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const uint32_t ch = w->child[c];
@@ -913,7 +926,7 @@ __device__ inline bool any_hit_wide(const palz_internal_node* __restrict__ nodes
                 } else if (sp == kPacketDepth) {
                     overflow = true;
                 } else {
-                    if (leader) {
+                    if (lane_in(first_lane())) {
                         stack->mask[sp] = mg;
                         stack->node[sp] = g;
                     }
@@ -930,8 +943,8 @@ __device__ inline bool any_hit_wide(const palz_internal_node* __restrict__ nodes
             while (sp > 0 && mask == 0ull) {
                 --sp;
                 __builtin_amdgcn_wave_barrier();
-                node = stack->node[sp];
-                mask = stack->mask[sp] & pending;
+                node = uniform32(stack->node[sp]);
+                mask = uniform64(stack->mask[sp]) & pending;
             }
             if (mask == 0ull) break;
         }
