@@ -61,18 +61,22 @@ typedef struct {
 } palz_internal_node;
 
 /* A two-level (BVH4) view of internal node i of an LBVH, for packet
- * traversal: its two children with their boxes (as in the node) and, for each
- * internal child c, that child's two children with their boxes at
- * grand[2c + j] / grand_box[2c + j] (PALZ_NODE_INVALID below a leaf child).
+ * traversal: its two children (as in the node) and, for each internal child
+ * c, that child's two children at grand[2c + j] (PALZ_NODE_INVALID below a
+ * leaf child), with the boxes of all six by plane: plane[a][0][s] is box s's
+ * lower bound on axis a, plane[a][1][s] its upper bound and plane[a][2][s]
+ * the lower bound again, for s = 0, 1 the children's boxes and s = 2 + 2c + j
+ * the grandchildren's (those below a leaf child hold 0).  So plane[a][0..1]
+ * and plane[a][1..2] are both (entry, exit) pairs: for rays whose direction
+ * component a is > 0 and < 0, a wave loads one or the other.
  * No counterpart in the reference: it holds what intersect_escape_link
  * (minimum_lbvh.h:1078-1152) reads from three nodes, so that one load serves
- * two levels.  176 B. */
+ * two levels.  248 B. */
 typedef struct {
     palz_node_index child[2];
     palz_node_index grand[4];
     uint32_t pad_[2];
-    palz_aabb box[2];
-    palz_aabb grand_box[4];
+    float plane[3][3][6];
 } palz_wide_node;
 
 /* Material (pk.h:67-72) and TriangleAttrib (pk.h:74-78): 40 B. */

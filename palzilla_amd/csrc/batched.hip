@@ -484,10 +484,17 @@ __device__ __forceinline__ void occlude_persistent(const palz_scene_desc* __rest
                     bool hit;
                     if (packet) {
                         const palz_scene_desc* us = scenes + scene0;
-                        hit = us->wide ? any_hit_wide(us->nodes, us->wide, us->tris, *us->root, visible, seg[i].ro,
-                                                      seg[i].rd, ird, 1.0f, pstack)
-                                       : any_hit_packet(us->nodes, us->tris, *us->root, visible, seg[i].ro,
-                                                        seg[i].rd, ird, 1.0f, pstack);
+                        if (us->wide) {
+                            const WaveSlabs ws = wave_slabs(ird, __ballot(visible));
+                            hit = ws.ordered ? any_hit_wide<true>(us->nodes, us->wide, us->tris, *us->root, visible,
+                                                                  seg[i].ro, seg[i].rd, ird, ws, 1.0f, pstack)
+                                             : any_hit_wide<false>(us->nodes, us->wide, us->tris, *us->root,
+                                                                   visible, seg[i].ro, seg[i].rd, ird, ws, 1.0f,
+                                                                   pstack);
+                        } else {
+                            hit = any_hit_packet(us->nodes, us->tris, *us->root, visible, seg[i].ro, seg[i].rd, ird,
+                                                 1.0f, pstack);
+                        }
                     } else {
                         hit = visible &&
                               any_hit_escape(sc->nodes, sc->tris, *sc->root, seg[i].ro, seg[i].rd, ird, 1.0f);

@@ -29,7 +29,7 @@ static_assert(offsetof(palz_internal_node, aabbs) == 8, "InternalNode layout");
 static_assert(offsetof(palz_internal_node, parent) == 56, "InternalNode layout");
 static_assert(offsetof(palz_internal_node, context) == 60, "InternalNode layout");
 static_assert(sizeof(palz_triangle_attrib) == 40, "TriangleAttrib layout (pk.h:74)");
-static_assert(sizeof(palz_wide_node) == 176, "wide node layout");
+static_assert(sizeof(palz_wide_node) == 248, "wide node layout");
 static_assert(sizeof(palz_normal_bound) == 52, "InternalNormalBound layout (pk.h:1705)");
 
 constexpr float kFltMax = 3.402823466e+38F;

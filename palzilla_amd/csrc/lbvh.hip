@@ -665,7 +665,7 @@ __global__ void __launch_bounds__(kGroupItems)
 }
 
 // palz_wide_node of every internal node: its record and its internal
-// children's records, merged.
+// children's records, merged, the six boxes stored by plane.
 __global__ void __launch_bounds__(256) k_widen(const palz_internal_node* __restrict__ nodes, uint32_t n_int,
                                                palz_wide_node* __restrict__ wide) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -673,19 +673,32 @@ __global__ void __launch_bounds__(256) k_widen(const palz_internal_node* __restr
     const palz_internal_node nd = nodes[i];
     palz_wide_node w;
     w.pad_[0] = w.pad_[1] = 0u;
+    palz_aabb box[6];
+    const palz_aabb none = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         w.child[c] = nd.children[c];
-        w.box[c] = nd.aabbs[c];
+        box[c] = nd.aabbs[c];
         if (nd.children[c] & PALZ_NODE_LEAF_BIT) {
             w.grand[2 * c] = w.grand[2 * c + 1] = PALZ_NODE_INVALID;
-            w.grand_box[2 * c] = w.grand_box[2 * c + 1] = palz_aabb{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+            box[2 + 2 * c] = box[3 + 2 * c] = none;
         } else {
             const palz_internal_node cn = nodes[nd.children[c]];
             w.grand[2 * c] = cn.children[0];
             w.grand[2 * c + 1] = cn.children[1];
-            w.grand_box[2 * c] = cn.aabbs[0];
-            w.grand_box[2 * c + 1] = cn.aabbs[1];
+            box[2 + 2 * c] = cn.aabbs[0];
+            box[3 + 2 * c] = cn.aabbs[1];
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const palz_aabb b = box[s];
+        const float lo[3] = {b.lower.x, b.lower.y, b.lower.z}, hi[3] = {b.upper.x, b.upper.y, b.upper.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            w.plane[a][0][s] = lo[a];
+            w.plane[a][1][s] = hi[a];
+            w.plane[a][2][s] = lo[a];
         }
     }
     wide[i] = w;

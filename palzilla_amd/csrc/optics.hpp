@@ -745,6 +745,21 @@ __device__ __forceinline__ T load_uniform(const T* p) {
     for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = q[i];
     return r;
 }
+// load_uniform of the record `byte_offset` bytes past `base` (a wave-uniform
+// offset, added in the constant address space so that it can ride in the
+// scalar load's offset register).
+template <class T>
+__device__ __forceinline__ T load_uniform_at(const void* base, uint32_t byte_offset) {
+    static_assert(sizeof(T) % 4 == 0, "dword records");
+    typedef const __attribute__((address_space(4))) char cbyte;
+    typedef const __attribute__((address_space(4))) uint32_t cword;
+    cword* q = (cword*)((cbyte*)base + byte_offset);
+    T r;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = q[i];
+    return r;
+}
 
 // Any-hit traversal of one occlusion ray per lane, run by the whole wave as a
 // packet: the wave walks the tree once, loading each node with scalar loads,
@@ -863,15 +878,62 @@ __device__ inline bool any_hit_packet(const palz_internal_node* __restrict__ nod
     return hit;
 }
 
+// The entry and exit planes, on one axis, of a wide node's six boxes (slots
+// as in palz_wide_node): plane[a][0..1] of the node for rays whose direction
+// component a is positive, plane[a][1..2] for negative ones.
+struct WidePlanes {
+    float entry[6], exit[6];
+};
+// Per wave and ray: the byte offset of each axis' WidePlanes in a wide node,
+// and whether every lane that queries has one direction sign per axis
+// (`ordered`: then the planes at those offsets are each lane's entry and exit
+// planes).
+struct WaveSlabs {
+    bool ordered;
+    uint32_t offset[3];
+};
+__device__ inline WaveSlabs wave_slabs(v3 ird, uint64_t lanes) {
+    const uint64_t neg[3] = {__ballot(ird.x < 0.0f) & lanes, __ballot(ird.y < 0.0f) & lanes,
+                             __ballot(ird.z < 0.0f) & lanes};
+    WaveSlabs w;
+    w.ordered = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        w.ordered = w.ordered && (neg[a] == 0ull || neg[a] == lanes);
+        w.offset[a] = (uint32_t)offsetof(palz_wide_node, plane) + (uint32_t)(a * 3 * 6 * sizeof(float)) +
+                      (neg[a] != 0ull ? (uint32_t)(6 * sizeof(float)) : 0u);
+    }
+    return w;
+}
+// slabs() for a ray whose direction signs are known: `nr` holds each axis'
+// entry plane (lo.x where ird.x > 0, hi.x where ird.x < 0) and `fr` its exit
+// plane.  On an axis with ird.x > 0, lo.x <= hi.x gives lo.x - ro.x <=
+// hi.x - ro.x, and (lo.x - ro.x) * ird.x <= (hi.x - ro.x) * ird.x since
+// rounding is monotone; so slabs()'s fold_min of the two is the entry plane's
+// t and its fold_max the exit plane's, up to the sign of a zero (inv_rd never
+// returns 0, and no t is NaN).  With ird.x < 0 the roles swap.  For boxes
+// with lower <= upper (every box the tree holds) the results compare as
+// slabs()'s do.
+PALZ_HD void slabs_ordered(v3 ro, v3 ird, v3 nr, v3 fr, float known_t, float* rmin, float* rmax) {
+    const v3 tn = (nr - ro) * ird;
+    const v3 tf = (fr - ro) * ird;
+    *rmin = fold_max(fold_max(fold_max(tn.x, tn.y), tn.z), 0.0f);
+    *rmax = fold_min(fold_min(fold_min(tf.x, tf.y), tf.z) * 1.00000024f, known_t);
+}
+
 // any_hit_packet over the BVH4 view (palz_wide_node): one scalar load brings
 // a node's two child boxes and its internal children's child boxes, so the
 // wave's walk takes one dependent load per two levels.  A lane still tests a
 // grandchild's box only if it hit the child's box, so each lane tests exactly
-// the leaves any_hit_escape would, and the result is the same.
+// the leaves any_hit_escape would, and the result is the same.  The boxes are
+// read by plane at the offsets of `ws`; kOrdered (ws.ordered) tests them with
+// slabs_ordered, else with slabs(), whose result does not depend on which of
+// the two planes of an axis comes first.
+template <bool kOrdered>
 __device__ inline bool any_hit_wide(const palz_internal_node* __restrict__ nodes,
                                     const palz_wide_node* __restrict__ wide, const palz_triangle* __restrict__ tris,
-                                    uint32_t root, bool query, v3 ro, v3 rd, v3 ird, float t_max,
-                                    PacketStack* stack) {
+                                    uint32_t root, bool query, v3 ro, v3 rd, v3 ird, const WaveSlabs& ws,
+                                    float t_max, PacketStack* stack) {
     const uint64_t me = 1ull << (threadIdx.x & 63);
     bool hit = false;
     uint64_t pending = __ballot(query);
@@ -890,24 +952,39 @@ __device__ inline bool any_hit_wide(const palz_internal_node* __restrict__ nodes
         test_leaf(root, pending);
         return hit;
     }
-    // every lane tests the box; the ballot keeps the lanes of `lanes`
-    auto box_hit = [&](const palz_aabb& b, uint64_t lanes) -> uint64_t {
-        return __ballot(box_test(ro, ird, b, t_max)) & lanes;
+    struct Head {
+        palz_node_index child[2];
+        palz_node_index grand[4];
+        uint32_t pad_[2];
     };
     int sp = 0;
     bool overflow = false;
     uint32_t node = root;
     uint64_t mask = pending;
     for (;;) {
-        const palz_wide_node wn = load_uniform(wide + __builtin_amdgcn_readfirstlane(node));
-        const palz_wide_node* w = &wn;
+        const palz_wide_node* at = wide + __builtin_amdgcn_readfirstlane(node);
+        const Head w = load_uniform(reinterpret_cast<const Head*>(at));
+        const WidePlanes px = load_uniform_at<WidePlanes>(at, ws.offset[0]);
+        const WidePlanes py = load_uniform_at<WidePlanes>(at, ws.offset[1]);
+        const WidePlanes pz = load_uniform_at<WidePlanes>(at, ws.offset[2]);
+        // every lane tests the box; the ballot keeps the lanes of `lanes`
+        auto box_hit = [&](int s, uint64_t lanes) -> uint64_t {
+            const v3 nr = mk(px.entry[s], py.entry[s], pz.entry[s]);
+            const v3 fr = mk(px.exit[s], py.exit[s], pz.exit[s]);
+            float a, c;
+            if (kOrdered)
+                slabs_ordered(ro, ird, nr, fr, t_max, &a, &c);
+            else
+                slabs(ro, ird, nr, fr, t_max, &a, &c);
+            return __ballot(a <= c) & lanes;
+        };
         // the first internal grandchild hit is walked next, the others pushed
         uint32_t first = PALZ_NODE_INVALID;
         uint64_t first_mask = 0ull;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            const uint32_t ch = w->child[c];
-            const uint64_t mc = box_hit(w->box[c], mask) & pending;
+            const uint32_t ch = w.child[c];
+            const uint64_t mc = box_hit(c, mask) & pending;
             if (mc == 0ull) continue;
             if (ch & PALZ_NODE_LEAF_BIT) {
                 test_leaf(ch, mc);
@@ -915,8 +992,8 @@ __device__ inline bool any_hit_wide(const palz_internal_node* __restrict__ nodes
             }
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                const uint32_t g = w->grand[2 * c + j];
-                const uint64_t mg = box_hit(w->grand_box[2 * c + j], mc & pending) & pending;
+                const uint32_t g = w.grand[2 * c + j];
+                const uint64_t mg = box_hit(2 + 2 * c + j, mc & pending) & pending;
                 if (mg == 0ull) continue;
                 if (g & PALZ_NODE_LEAF_BIT) {
                     test_leaf(g, mg);

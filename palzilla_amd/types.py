@@ -16,10 +16,10 @@ INTERNAL_NODE = np.dtype([("children", "<u4", 2), ("aabbs", AABB, 2), ("parent",
                           ("context", "<u4")])
 assert INTERNAL_NODE.itemsize == 64
 
-# palz_wide_node: the BVH4 view of one internal node, 176 B
-WIDE_NODE = np.dtype([("child", "<u4", 2), ("grand", "<u4", 4), ("pad", "<u4", 2), ("box", AABB, 2),
-                      ("grand_box", AABB, 4)])
-assert WIDE_NODE.itemsize == 176
+# palz_wide_node: the BVH4 view of one internal node, its six boxes by plane:
+# plane[axis][lower, upper, lower][slot], 248 B
+WIDE_NODE = np.dtype([("child", "<u4", 2), ("grand", "<u4", 4), ("pad", "<u4", 2), ("plane", "<f4", (3, 3, 6))])
+assert WIDE_NODE.itemsize == 248
 
 # InternalNormalBound (pk.h:1705-1709): 2 AABBs of shading normals + arrival counter, 52 B
 NORMAL_BOUND = np.dtype([("normal_bounds", AABB, 2), ("counter", "<u4")])

@@ -162,19 +162,33 @@ def test_fixed_point_image_is_exact_and_order_independent(run):
     assert np.allclose(img, want_img, rtol=1e-5, atol=1e-6)
 
 
+def _wide_boxes(w):
+    """(lower, upper) of a wide node's six box slots, each (n, 6, 3), from
+    its planes; the third plane row repeats the lower bounds."""
+    pl = w["plane"]  # (n, axis, row, slot)
+    assert pl[:, :, 2, :].tobytes() == pl[:, :, 0, :].tobytes()
+    return np.ascontiguousarray(pl[:, :, 0, :].transpose(0, 2, 1)), \
+        np.ascontiguousarray(pl[:, :, 1, :].transpose(0, 2, 1))
+
+
 def test_wide_nodes_mirror_the_tree(run):
     n_int = run.s.n_tris - 1
     w = download_ptr(run.g.wide, T.WIDE_NODE, n_int)
     nodes = run.nodes
+    lo, hi = _wide_boxes(w)
     assert np.array_equal(w["child"], nodes["children"])
-    assert w["box"].tobytes() == nodes["aabbs"].tobytes()
+    assert lo[:, 0:2].tobytes() == np.ascontiguousarray(nodes["aabbs"]["lower"]).tobytes()
+    assert hi[:, 0:2].tobytes() == np.ascontiguousarray(nodes["aabbs"]["upper"]).tobytes()
     for c in range(2):
         ch = nodes["children"][:, c]
         internal = (ch & 0x80000000) == 0
         idx = ch[internal]
+        s = slice(2 + 2 * c, 4 + 2 * c)
         assert np.array_equal(w["grand"][internal, 2 * c:2 * c + 2], nodes["children"][idx])
-        assert w["grand_box"][internal, 2 * c:2 * c + 2].tobytes() == nodes["aabbs"][idx].tobytes()
+        assert lo[internal, s].tobytes() == np.ascontiguousarray(nodes["aabbs"]["lower"][idx]).tobytes()
+        assert hi[internal, s].tobytes() == np.ascontiguousarray(nodes["aabbs"]["upper"][idx]).tobytes()
         assert np.all(w["grand"][~internal, 2 * c:2 * c + 2] == 0x7FFFFFFF)
+        assert np.all(lo[~internal, s] == 0) and np.all(hi[~internal, s] == 0)
 
 
 def test_wide_walk_matches_node_walk(run):
