@@ -373,16 +373,17 @@ __device__ __forceinline__ void refine_step(const float* prm, v3 pb, v3 pe, cons
                                             uint32_t event, double* p64) {
     constexpr int P = 2 * K;
     constexpr int R = 3 * K;
-    float A32[R * P];
-    float b32[R];
-    float cost32 = 0.0f;
-    chain_residual<K, P, float, v3>(prm, 0, pb, pe, tri, eta, event, A32, b32, &cost32);
 #pragma unroll
     for (int i = 0; i < P; ++i) p64[i] = (double)prm[i];
+    // the FP64 residual first: only it is live through the FP32 pass
     double A[R * P];
     double b[R];
     double cost = 0.0;
     chain_residual<K, 0, double, d3>(p64, 0, mkd(pb), mkd(pe), tri, (double)eta, event, A, b, &cost);
+    float A32[R * P];
+    float b32[R];
+    float cost32 = 0.0f;
+    chain_residual<K, P, float, v3>(prm, 0, pb, pe, tri, eta, event, A32, b32, &cost32);
 #pragma unroll
     for (int i = 0; i < R * P; ++i) A[i] = (double)A32[i];
     double dp[P];
@@ -571,27 +572,29 @@ __global__ void __launch_bounds__(256)
         float pd[P];  // the chain dA/dw and the receiver cosine take
 #pragma unroll
         for (int i = 0; i < P; ++i) pf[i] = pd[i] = prm[i];
-        if (kMode != PALZ_REFINE_NONE) {
+        // The FP64 step: before dA/dw when dA/dw takes its chain (ALL), else
+        // after it, so that the step's registers and dA/dw's are not live at
+        // once.
+        auto refine = [&]() {
             double p64[P];
             refine_step<K>(prm, pb, pe, tri, eta, event, p64);
 #pragma unroll
             for (int i = 0; i < P; ++i) pf[i] = (float)p64[i];
-            if (kMode == PALZ_REFINE_ALL) {
-#pragma unroll
-                for (int i = 0; i < P; ++i) pd[i] = pf[i];
-            }
             if (refined) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) refined[(size_t)item * 8 + i] = i < P ? p64[i] : 0.0;
             }
+        };
+        if (kMode == PALZ_REFINE_ALL) {
+            refine();
+#pragma unroll
+            for (int i = 0; i < P; ++i) pd[i] = pf[i];
         }
         palz_path_result r;
         r.status = PALZ_PATH_CONVERGED;
         r.iterations = conv_iters[item];
         r.contribution = 0.0f;
         r.footprint = 0.0f;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) r.params[i] = i < P ? pf[i] : 0.0f;
         if (0.0f < thr) {
             const float area = dadw<K>(pb, chain_vertex(tri[0], pd[0], pd[1]) - pb, pe, tri, event, eta);
             const v3 last = chain_vertex(tri[K - 1], pd[P - 2], pd[P - 1]);
@@ -603,6 +606,9 @@ __global__ void __launch_bounds__(256)
             r.footprint = area;
             ++n_contrib;
         }
+        if (kMode == PALZ_REFINE_VERTICES) refine();
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r.params[i] = i < P ? pf[i] : 0.0f;
         if (results) results[item] = r;
     }
     wave_add_stat(&ctr->stats[(K - 1) * 4 + 2], n_contrib);
