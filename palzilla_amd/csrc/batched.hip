@@ -395,12 +395,14 @@ __device__ __forceinline__ void refine_step(const float* prm, v3 pb, v3 pe, cons
 // contributableThroughput (pk.h:760-842) of the converged chains of length K,
 // in item order: every test but the occlusion rays per lane, then the K + 1
 // occlusion rays as wave packets (per lane when the wave's items belong to
-// more than one scene or to more than kPacketPaths paths).  Writes the throughput (0 when any test fails) per
-// list position.  A kernel of its own so that the latency-bound traversal runs
-// at the occupancy of its few live registers, not at that of dA/dw's.
+// more than one scene).  Writes the throughput (0 when any test fails) per
+// list position.  A kernel of its own so that the traversal runs at the
+// occupancy of its few live registers, not at that of dA/dw's.
 //
-// The list keeps the tuple-coherent path order, so neighbouring positions
-// trace neighbouring rays.  Each XCD (the workgroups with blockIdx % 8 equal)
+// The list keeps the batch order (chain length, scene, first triangle), so
+// neighbouring positions trace neighbouring rays: a wave's packet walk visits
+// few more nodes than one of its rays, whether its items are the wavelengths
+// of a few paths or 64 paths of one wavelength.  Each XCD (the workgroups with blockIdx % 8 equal)
 // takes a contiguous eighth of the positions, so that its L2 holds the nodes
 // and triangles of one part of the scene rather than of all of it; its waves
 // take kOccludeGrab positions at a time from that eighth's counter, then help
@@ -408,8 +410,6 @@ __device__ __forceinline__ void refine_step(const float* prm, v3 pb, v3 pe, cons
 // passes): large enough to keep the counters cold on big lists, small enough
 // that a short list still spreads over every wave.
 constexpr uint32_t kOccludeGrabMax = 256;
-// Most distinct paths in a wave for which the occlusion rays walk as one packet.
-constexpr uint32_t kPacketPaths = 16;
 
 // Clears chain length K's per-XCD counters before its k_occlude, or with
 // K = 0 every length's (a kernel, not a memset, so that it can be captured in
@@ -469,15 +469,7 @@ __device__ __forceinline__ void occlude_persistent(const palz_scene_desc* __rest
                     thr = throughput_unoccluded<K>(prm, pb, pe, tri, path.event, item_eta(sc, l), seg);
                 }
                 const uint32_t scene0 = __builtin_amdgcn_readfirstlane(path.scene);
-                const bool one_scene = __ballot(path.scene != scene0) == 0ull;
-                // A packet walk visits the union of its lanes' nodes: cheap when
-                // the wave's items are the wavelengths of a few paths (their rays
-                // nearly coincide), costly when the lanes hold many different
-                // paths (one wavelength per path).  Those waves walk per lane.
-                const uint32_t prev_pi = (uint32_t)__shfl_up((int)pi, 1, 64);
-                const uint32_t n_paths_wave =
-                    (uint32_t)__popcll(__ballot(in_range && (lane == 0u || prev_pi != pi)));
-                const bool packet = one_scene && n_paths_wave <= kPacketPaths;
+                const bool packet = __ballot(path.scene != scene0) == 0ull;  // one scene
                 bool visible = 0.0f < thr;
 #pragma unroll
                 for (int i = 0; i < K + 1; ++i) {
@@ -509,7 +501,7 @@ __device__ __forceinline__ void occlude_persistent(const palz_scene_desc* __rest
 }
 
 template <int K>
-__global__ void __launch_bounds__(256, 8)  // 8 waves per SIMD: latency-bound, few live registers
+__global__ void __launch_bounds__(256, 8)  // 8 waves per SIMD: few live registers
     k_occlude(const palz_scene_desc* __restrict__ scenes, const palz_path* __restrict__ paths,
               const uint32_t* __restrict__ ranges, uint32_t L, Counters* __restrict__ ctr,
               const uint32_t* __restrict__ excl, const uint32_t* __restrict__ list,

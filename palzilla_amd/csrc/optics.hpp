@@ -182,21 +182,25 @@ PALZ_HD dual3<NT, F> drefraction(const dual3<NT, F>& wi, const dual3<NT, F>& n, 
 
 // ---------------------------------------------------------------------------
 // Householder QR least squares of a rows x cols system (sen.h:658-747).
-// A is column-major: A[col * rows + row].
+// A is column-major: A[col * rows + row].  The reference keeps every
+// Householder vector and applies them to b after the factorisation
+// (applyQTransposed, sen.h:618-647); here each is applied to b as soon as it
+// is formed, so that only one vector is live.  b's updates read only that
+// vector and b, and run in the same order, so the arithmetic is the same.
 template <int R, int C, class F>
 PALZ_HD void solve_qr(F* A, F* b, F* x) {
     constexpr int H = (C < R - 1) ? C : R - 1;
-    F vs[R * H];
 #pragma unroll
     for (int i = 0; i < H; ++i) {
+        F v[R];
 #pragma unroll
-        for (int j = 0; j < R; ++j) vs[i * R + j] = j < i ? F(0) : -A[i * R + j];
-        const F sgn = F(0) < vs[i * R + i] ? F(1) : F(-1);
+        for (int j = 0; j < R; ++j) v[j] = j < i ? F(0) : -A[i * R + j];
+        const F sgn = F(0) < v[i] ? F(1) : F(-1);
         F cd = F(0);
 #pragma unroll
-        for (int j = 0; j < R; ++j) cd += vs[i * R + j] * vs[i * R + j];
+        for (int j = 0; j < R; ++j) cd += v[j] * v[j];
         const F x_len = sqrt(cd);
-        vs[i * R + i] += sgn * x_len;
+        v[i] += sgn * x_len;
         const F v_len2 = F(2) * x_len * (x_len + fabs(A[i * R + i]));
         A[i * R + i] = sgn * x_len;
 #pragma unroll
@@ -205,25 +209,21 @@ PALZ_HD void solve_qr(F* A, F* b, F* x) {
         for (int c = i + 1; c < C; ++c) {
             F vdc = F(0);
 #pragma unroll
-            for (int r = i; r < R; ++r) vdc += vs[i * R + r] * A[c * R + r];
+            for (int r = i; r < R; ++r) vdc += v[r] * A[c * R + r];
             const F k = F(2) * vdc / v_len2;
 #pragma unroll
-            for (int r = i; r < R; ++r) A[c * R + r] -= k * vs[i * R + r];
+            for (int r = i; r < R; ++r) A[c * R + r] -= k * v[r];
         }
-    }
-    // applyQTransposed (sen.h:618-647)
-#pragma unroll
-    for (int i = 0; i < H; ++i) {
+        // applyQTransposed's step i
         F vdc = F(0), vdv = F(0);
 #pragma unroll
         for (int r = i; r < R; ++r) {
-            const F v = vs[i * R + r];
-            vdc += v * b[r];
-            vdv += v * v;
+            vdc += v[r] * b[r];
+            vdv += v[r] * v[r];
         }
         const F k = F(2) * vdc / vdv;
 #pragma unroll
-        for (int r = i; r < R; ++r) b[r] -= k * vs[i * R + r];
+        for (int r = i; r < R; ++r) b[r] -= k * v[r];
     }
     // back substitution (sen.h:736-744)
 #pragma unroll
