@@ -130,6 +130,23 @@ def profiled_traffic(workload: str, kernel: str):
     return None
 
 
+def profiled_insts(workload: str, kernel: str):
+    """Executed instructions per dispatch of `kernel` from the committed
+    rocprofv3 PMC summary (SQ_INSTS_VALU, SQ_INSTS_SALU, SQ_INSTS_SMEM,
+    SQ_INSTS_BRANCH: wave instructions), or None."""
+    path = os.path.join(ROOT, "profiles", "solver_instruction_mix_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 6 and cells[0] == workload and cells[1] == f"`{kernel}`":
+            try:
+                return {k: float(v) * 1e9 for k, v in zip(("valu", "salu", "smem", "branch"), cells[2:6])}
+            except ValueError:
+                return None
+    return None
+
+
 def _solver_flops(stats: dict, event_share: dict) -> dict:
     """Algorithmic FLOPs of the Newton kernels per chain length: residual
     evaluations and steps times the reference iteration's counts, weighted by
@@ -397,6 +414,22 @@ def bench_c2(args) -> dict:
                      "units": {f"K{K}": {"paths": n_p, "bytes_per_path": per_p, "items": n_i, "bytes_per_item": per_i}
                                for K, n_p, per_p, n_i, per_i in dom_units}})
     roof["traffic"] = profiled_traffic("c2", name)
+    if dominant.startswith("newton"):
+        # The Newton kernels are compute-bound (their HBM rate is the `hbm`
+        # entry below): the roof is the FP32 peak over the reference
+        # iteration's FLOPs.  The path has no matrix-shaped work, so the FP32
+        # VALU pipe is the one that runs; its executed instructions per
+        # dispatch come from the committed PMC summary.
+        f = sum(flops[K] for K, *_ in dom_units)
+        tflops = f / (dom_ms * 1e-3) / 1e12
+        roof["hbm"] = {k: roof.pop(k) for k in ("achieved", "peak", "unit", "frac", "algorithmic_bytes") if k in roof}
+        roof.update({"bound": "mfma", "pipe": "FP32 VALU", "achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "algorithmic_flops": f,
+                     "counted": "reference-equivalent FLOPs (2K single-tangent passes per iteration)"})
+        insts = profiled_insts("c2", name)
+        if insts:
+            roof["executed"] = {"valu_wave_insts": insts["valu"], "salu_wave_insts": insts["salu"],
+                                "valu_G_per_s": round(insts["valu"] / (dom_ms * 1e-3) / 1e9, 1)}
     # the occlusion kernels are traversal-bound: rays per second and the
     # escape-link walk's work per ray (oracle counts on the sample)
     ow = sample.get("occlusion_work")
