@@ -12,7 +12,9 @@
 //   dA/dw ............... pk.h:844-930
 //   traversal ........... minimum_lbvh.h:82-96, :372-422, :927-988, :1078-1152
 //   Sobol ............... sobol.h:17-129
-// Host min/max semantics (helper_math.h:134-142) are used everywhere.
+// Host min/max semantics (helper_math.h:134-142) are used everywhere but in
+// the slab test's folds (fminf/fmaxf, whose box decisions are the same: see
+// slabs; tests/slab_check.cpp checks them on the CPU).
 #pragma once
 
 #include "common.hpp"
