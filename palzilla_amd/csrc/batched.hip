@@ -718,11 +718,15 @@ struct palz_batched {
     float* thr = nullptr;  // per list position: the throughput after the occlusion rays
     uint32_t* scan_sums = nullptr;
     size_t cap_items = 0;
-    palz_launch_config cfg[3] = {{256, 4, 256, 128}, {256, 2, 256, 128}, {128, 2, 256, 128}};
+    // Shapes until set or tuned: the autotuner's choice for a full-GPU list of
+    // 16 wavelengths per path (the spectral slab), which fills every SIMD with
+    // waves: the Newton kernels at their register budgets (5 / 3 / 2 waves per
+    // SIMD for K = 1 / 2 / 3), the occlusion kernels at 8 (256-lane blocks).
+    palz_launch_config cfg[3] = {{256, 8, 1024, 256}, {256, 4, 1024, 256}, {128, 8, 256, 256}};
     // the mixed-length kernels' shape (block, blocks_per_cu, the Newton
     // kernel's register budget in waves per SIMD (2 or 3), occlusion block);
     // each length keeps its own queue chunk from cfg
-    palz_launch_config cfg_mixed = {256, 2, 2, 256};
+    palz_launch_config cfg_mixed = {256, 4, 3, 256};
     // fused mode, a bit mask: 1 = one persistent Newton kernel over every chain
     // length, 2 = one occlusion kernel over every chain length (3 = both, 0 =
     // one of each per length); -1 = chosen by the autotuner (per length until

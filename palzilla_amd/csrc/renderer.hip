@@ -195,7 +195,9 @@ int palz_renderer_create(uint32_t width, uint32_t height, const float* lambdas, 
         return rc == PALZ_OK;
     };
     ok(palz_sorter_create(&r->sorter)) && ok(palz_lbvh_create(&r->lbvh)) && ok(palz_tuple_table_create(&r->table)) &&
-        ok(palz_batched_create(&r->batched)) && alloc((void**)&r->d_points, sizeof(palz_first_diffuse) * np) &&
+        ok(palz_batched_create(&r->batched)) &&
+        ok(palz_batched_set_fused(r->batched, 2)) &&  // one occlusion kernel over every chain length
+        alloc((void**)&r->d_points, sizeof(palz_first_diffuse) * np) &&
         alloc((void**)&r->d_accum, sizeof(palz_float4) * np) && alloc((void**)&r->d_frame, sizeof(palz_float4) * np) &&
         alloc((void**)&r->d_pixels, sizeof(uint32_t) * np) &&
         alloc((void**)&r->d_images, sizeof(float) * np * n_lambdas) &&
