@@ -37,7 +37,7 @@ import numpy as np  # noqa: E402
 METRIC = "converged specular paths/s at 1e-5 vertex err, batched vs plain; LBVH+sort Mtri/s"
 HBM_PEAK_GBPS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-CPU_THREADS = 16  # the GPU box's host-core share for one GPU
+CPU_THREADS = 16  # host cores per GPU on the benchmark machine
 
 # Algorithmic bytes per triangle of each LBVH stage (the reference's pass
 # structure): AABB read 36; Morton read 36 + write 8; sort pass read 8 + write
