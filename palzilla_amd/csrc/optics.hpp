@@ -544,9 +544,46 @@ __host__ __device__ inline bool any_hit_escape(const palz_internal_node* nodes, 
 // stack lives in LDS as stack[depth * stride + lane] (one column per lane, so
 // pushes and pops of a wave hit distinct banks).  Returns false if the stack
 // would overflow `depth` entries; the hit is then incomplete.
+// The traversal stack of closest_hit: one lane's entries interleaved with its
+// block's other lanes (entry i at stack[i * stride]).
+struct StridedStack {
+    uint32_t* base;
+    int stride;
+    PALZ_HD void put(int i, uint32_t v) { base[i * stride] = v; }
+    PALZ_HD uint32_t get(int i) const { return base[i * stride]; }
+};
+// A stack whose first kNear entries are a StridedStack (in LDS) and the rest a
+// private array: deep entries are rare, so the LDS per lane (and with it the
+// block's LDS, which bounds how many waves a CU holds) stays small.
+template <int kNear, int kFar>
+struct SplitStack {
+    StridedStack near;
+    uint32_t far[kFar];
+    PALZ_HD void put(int i, uint32_t v) {
+        if (i < kNear)
+            near.put(i, v);
+        else
+            far[i - kNear] = v;
+    }
+    PALZ_HD uint32_t get(int i) const { return i < kNear ? near.get(i) : far[i - kNear]; }
+};
+
+template <class Stack>
+__host__ __device__ inline bool closest_hit_on(Hit* hit, const palz_internal_node* nodes, const palz_triangle* tris,
+                                               uint32_t root, v3 ro, v3 rd, v3 ird, Stack& stack, int depth);
+
 __host__ __device__ inline bool closest_hit(Hit* hit, const palz_internal_node* nodes, const palz_triangle* tris,
                                    uint32_t root, v3 ro, v3 rd, v3 ird, uint32_t* stack, int stride,
                                    int depth) {
+    StridedStack st{stack, stride};
+    return closest_hit_on(hit, nodes, tris, root, ro, rd, ird, st, depth);
+}
+
+// intersect_stackfull (minimum_lbvh.h:927-988) on any stack type: near child
+// first, the far one pushed; false when `depth` entries would not suffice.
+template <class Stack>
+__host__ __device__ inline bool closest_hit_on(Hit* hit, const palz_internal_node* nodes, const palz_triangle* tris,
+                                               uint32_t root, v3 ro, v3 rd, v3 ird, Stack& stack, int depth) {
     int sp = 0;
     uint32_t node = root;
     while (node != PALZ_NODE_INVALID) {
@@ -562,7 +599,7 @@ __host__ __device__ inline bool closest_hit(Hit* hit, const palz_internal_node* 
                 hit->ng = ng;
                 hit->tri = ti;
             }
-            node = sp == 0 ? PALZ_NODE_INVALID : stack[(--sp) * stride];
+            node = sp == 0 ? PALZ_NODE_INVALID : stack.get(--sp);
             continue;
         }
         const palz_internal_node& nd = nodes[node];
@@ -579,11 +616,11 @@ __host__ __device__ inline bool closest_hit(Hit* hit, const palz_internal_node* 
             }
             if (sp >= depth) return false;
             node = near_node;
-            stack[(sp++) * stride] = far_node;
+            stack.put(sp++, far_node);
         } else if (hit_l || hit_r) {
             node = nd.children[hit_l ? 0 : 1];
         } else {
-            node = sp == 0 ? PALZ_NODE_INVALID : stack[(--sp) * stride];
+            node = sp == 0 ? PALZ_NODE_INVALID : stack.get(--sp);
         }
     }
     return true;

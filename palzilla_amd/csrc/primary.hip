@@ -6,7 +6,8 @@ namespace palz {
 namespace {
 
 constexpr int kPrimaryThreads = 128;
-constexpr int kPrimaryStack = 64;
+constexpr int kPrimaryStack = 64;     // traversal stack entries per pixel
+constexpr int kPrimaryStackLds = 16;  // of which in LDS; the rest in private memory
 constexpr int kMaxBounces = 32;
 
 // sphIntersect (pk.h:52-65)
@@ -61,8 +62,9 @@ __device__ v3 sample_point(const uint8_t* tex, uint32_t w, uint32_t h, float u, 
 __global__ void __launch_bounds__(kPrimaryThreads)
     k_solve_primary(palz_scene_view sc, palz_camera cam, palz_primary_params pp,
                     palz_first_diffuse* __restrict__ out, float4* __restrict__ accum) {
-    __shared__ uint32_t s_stack[kPrimaryStack * kPrimaryThreads];
-    uint32_t* stack = s_stack + threadIdx.x;
+    __shared__ uint32_t s_stack[kPrimaryStackLds * kPrimaryThreads];
+    SplitStack<kPrimaryStackLds, kPrimaryStack - kPrimaryStackLds> stack;
+    stack.near = StridedStack{s_stack + threadIdx.x, kPrimaryThreads};
     const uint32_t pixel = blockIdx.x * kPrimaryThreads + threadIdx.x;
     if (pixel >= pp.width * pp.height) return;
     const uint32_t xi = pixel % pp.width, yi = pixel / pp.width;
@@ -94,7 +96,7 @@ __global__ void __launch_bounds__(kPrimaryThreads)
         hit.u = hit.v = 0.0f;
         hit.ng = mk(0.0f, 0.0f, 0.0f);
         hit.tri = 0xFFFFFFFFu;
-        if (!closest_hit(&hit, sc.nodes, sc.tris, root, ro, rd, inv_rd(rd), stack, kPrimaryThreads, kPrimaryStack))
+        if (!closest_hit_on(&hit, sc.nodes, sc.tris, root, ro, rd, inv_rd(rd), stack, kPrimaryStack))
             break;  // the stack would overflow: no hit is reported
         if (hit.t == kFltMax) break;
         const palz_triangle_attrib at = sc.attribs[hit.tri];

@@ -17,12 +17,13 @@ namespace palz {
 namespace {
 
 constexpr int kRayThreads = 128;
-constexpr int kRayStack = 64;
+constexpr int kRayStack = 64;     // traversal stack entries per ray
+constexpr int kRayStackLds = 16;  // of which in LDS; the rest in private memory
 
 __global__ void __launch_bounds__(kRayThreads)
     k_intersect_rays(palz_scene_view sc, const float* __restrict__ ro3, const float* __restrict__ rd3, uint32_t n,
                      int query, float t_max, palz_hit* __restrict__ out) {
-    __shared__ uint32_t s_stack[kRayStack * kRayThreads];
+    __shared__ uint32_t s_stack[kRayStackLds * kRayThreads];
     const uint32_t i = blockIdx.x * kRayThreads + threadIdx.x;
     if (i >= n) return;
     const v3 ro = mk(ro3[3 * i], ro3[3 * i + 1], ro3[3 * i + 2]);
@@ -36,7 +37,9 @@ __global__ void __launch_bounds__(kRayThreads)
     hit.tri = 0xFFFFFFFFu;
     if (query == PALZ_QUERY_CLOSEST) {
         Hit h = hit;
-        if (closest_hit(&h, sc.nodes, sc.tris, root, ro, rd, ird, s_stack + threadIdx.x, kRayThreads, kRayStack))
+        SplitStack<kRayStackLds, kRayStack - kRayStackLds> stack;
+        stack.near = StridedStack{s_stack + threadIdx.x, kRayThreads};
+        if (closest_hit_on(&h, sc.nodes, sc.tris, root, ro, rd, ird, stack, kRayStack))
             hit = h;
         else
             closest_hit_stackfree(&hit, sc.nodes, sc.tris, root, ro, rd, ird, false);

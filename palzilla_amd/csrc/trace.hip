@@ -28,7 +28,8 @@ namespace palz {
 namespace {
 
 constexpr int kTraceThreads = 128;
-constexpr int kTraceStack = 64;
+constexpr int kTraceStack = 64;      // traversal stack entries per photon
+constexpr int kTraceStackLds = 16;   // of which in LDS; the rest in private memory
 constexpr int kCellBits = 14;
 constexpr int kTupleBits = 22;
 constexpr uint32_t kCellMax = (1u << kCellBits) - 1u;
@@ -39,8 +40,9 @@ enum : uint32_t { kStatusCellRange = 1u, kStatusTupleCount = 2u, kStatusHashColl
 __global__ void __launch_bounds__(kTraceThreads)
     k_photon_trace(palz_scene_view sc, palz_photon_params pp, palz_photon_record* __restrict__ out,
                    uint32_t* __restrict__ status) {
-    __shared__ uint32_t s_stack[kTraceStack * kTraceThreads];
-    uint32_t* stack = s_stack + threadIdx.x;
+    __shared__ uint32_t s_stack[kTraceStackLds * kTraceThreads];
+    SplitStack<kTraceStackLds, kTraceStack - kTraceStackLds> stack;
+    stack.near = StridedStack{s_stack + threadIdx.x, kTraceThreads};
     const uint64_t g = (uint64_t)blockIdx.x * kTraceThreads + threadIdx.x;
     const uint32_t P = pp.photons_per_triangle;
     if (g >= (uint64_t)sc.n_tris * P) return;
@@ -90,8 +92,7 @@ __global__ void __launch_bounds__(kTraceThreads)
                 hit.u = hit.v = 0.0f;
                 hit.ng = mk(0.0f, 0.0f, 0.0f);
                 hit.tri = 0xFFFFFFFFu;
-                if (!closest_hit(&hit, sc.nodes, sc.tris, root, ro, rd, inv_rd(rd), stack, kTraceThreads,
-                                 kTraceStack)) {
+                if (!closest_hit_on(&hit, sc.nodes, sc.tris, root, ro, rd, inv_rd(rd), stack, kTraceStack)) {
                     if (status) atomicOr(status, 1u << 8);
                     admissible = false;
                     break;
