@@ -177,33 +177,53 @@ __device__ __forceinline__ bool same_tuple(const palz_photon_record& a, const pa
 __global__ void __launch_bounds__(256) k_tuple_hash(const palz_photon_record* __restrict__ rec, uint32_t n,
                                                     uint64_t* __restrict__ words, uint32_t* __restrict__ vals,
                                                     uint32_t* __restrict__ n_valid) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const palz_photon_record r = rec[i];
-    const bool valid = r.event != 0u;
-    words[i] = valid ? tuple_hash(r) : ~0ull;
-    vals[i] = i;
-    const uint64_t vm = __ballot(valid);
-    if ((threadIdx.x & 63) == __ffsll((unsigned long long)vm) - 1) atomicAdd(n_valid, (uint32_t)__popcll(vm));
+    // grid-stride, valid photons counted per block: one atomic per block
+    __shared__ uint32_t s_count;
+    if (threadIdx.x == 0) s_count = 0u;
+    __syncthreads();
+    uint32_t count = 0u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const palz_photon_record r = rec[i];
+        const bool valid = r.event != 0u;
+        words[i] = valid ? tuple_hash(r) : ~0ull;
+        vals[i] = i;
+        count += valid ? 1u : 0u;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) count += (uint32_t)__shfl_xor((int)count, o, 64);
+    if ((threadIdx.x & 63) == 0 && count) atomicAdd(&s_count, count);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_count) atomicAdd(n_valid, s_count);
 }
 
-// Smallest cell coordinate over the landing points (the key origin).
+// The smallest cell of the valid photons' end points per axis (the key
+// origin): a grid-stride loop reduced per block, one atomic per block and axis
+// (atomics from every wave on the same three words serialise at the L2).
 __global__ void __launch_bounds__(256) k_cell_min(const palz_photon_record* __restrict__ rec, uint32_t n,
                                                   float step, int* __restrict__ cell_min) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int s_min[3][256 / 64];
     int m[3] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF};
-    if (i < n && rec[i].event != 0u) {
-        const palz_float3 p = rec[i].p_final;
-        m[0] = (int)floorf(p.x / step);
-        m[1] = (int)floorf(p.y / step);
-        m[2] = (int)floorf(p.z / step);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (rec[i].event != 0u) {
+            const palz_float3 p = rec[i].p_final;
+            m[0] = min(m[0], (int)floorf(p.x / step));
+            m[1] = min(m[1], (int)floorf(p.y / step));
+            m[2] = min(m[2], (int)floorf(p.z / step));
+        }
     }
+    const uint32_t wave = threadIdx.x >> 6;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         int v = m[c];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-        if ((threadIdx.x & 63) == 0 && v != 0x7FFFFFFF) atomicMin(cell_min + c, v);
+        if ((threadIdx.x & 63) == 0) s_min[c][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int v = 0x7FFFFFFF;
+        for (uint32_t w = 0; w < blockDim.x / 64u; ++w) v = min(v, s_min[threadIdx.x][w]);
+        if (v != 0x7FFFFFFF) atomicMin(cell_min + threadIdx.x, v);
     }
 }
 
@@ -514,8 +534,8 @@ int palz_tuple_table_build(palz_tuple_table* t, const palz_photon_record* d_reco
     uint32_t* n_raw = t->scalars + 5;  // 8 * n_valid
     k_init_table<<<1, 64, 0, st>>>(t->scalars, t->cell_min);
     const uint32_t nb = div_up(cap, 256u);
-    k_tuple_hash<<<nb, 256, 0, st>>>(d_records, n_records, t->words, t->vals, n_valid);
-    k_cell_min<<<nb, 256, 0, st>>>(d_records, n_records, cell_step, t->cell_min);
+    k_tuple_hash<<<std::min(nb, 1024u), 256, 0, st>>>(d_records, n_records, t->words, t->vals, n_valid);
+    k_cell_min<<<std::min(nb, 1024u), 256, 0, st>>>(d_records, n_records, cell_step, t->cell_min);
     k_scale<<<1, 1, 0, st>>>(n_valid, 8u, n_raw);
     const SortScratch& s = sorter->scratch;
     // 1. tuples by content hash; stable, so equal tuples keep photon order.
