@@ -395,14 +395,13 @@ __device__ __forceinline__ void refine_step(const float* prm, v3 pb, v3 pe, cons
 // contributableThroughput (pk.h:760-842) of the converged chains of length K,
 // in item order: every test but the occlusion rays per lane, then the K + 1
 // occlusion rays as wave packets (per lane when the wave's items belong to
-// more than one scene).  Writes the throughput (0 when any test fails) per
+// more than one scene, and past the first segment when they belong to more
+// than kPacketPaths paths).  Writes the throughput (0 when any test fails) per
 // list position.  A kernel of its own so that the traversal runs at the
 // occupancy of its few live registers, not at that of dA/dw's.
 //
 // The list keeps the batch order (chain length, scene, first triangle), so
-// neighbouring positions trace neighbouring rays: a wave's packet walk visits
-// few more nodes than one of its rays, whether its items are the wavelengths
-// of a few paths or 64 paths of one wavelength.  Each XCD (the workgroups with blockIdx % 8 equal)
+// neighbouring positions trace neighbouring rays.  Each XCD (the workgroups with blockIdx % 8 equal)
 // takes a contiguous eighth of the positions, so that its L2 holds the nodes
 // and triangles of one part of the scene rather than of all of it; its waves
 // take kOccludeGrab positions at a time from that eighth's counter, then help
@@ -410,6 +409,8 @@ __device__ __forceinline__ void refine_step(const float* prm, v3 pb, v3 pe, cons
 // passes): large enough to keep the counters cold on big lists, small enough
 // that a short list still spreads over every wave.
 constexpr uint32_t kOccludeGrabMax = 256;
+// Most distinct paths in a wave whose segments past the first walk as packets.
+constexpr uint32_t kPacketPaths = 16;
 
 // Clears chain length K's per-XCD counters before its k_occlude, or with
 // K = 0 every length's (a kernel, not a memset, so that it can be captured in
@@ -469,12 +470,21 @@ __device__ __forceinline__ void occlude_persistent(const palz_scene_desc* __rest
                     thr = throughput_unoccluded<K>(prm, pb, pe, tri, path.event, item_eta(sc, l), seg);
                 }
                 const uint32_t scene0 = __builtin_amdgcn_readfirstlane(path.scene);
-                const bool packet = __ballot(path.scene != scene0) == 0ull;  // one scene
+                const bool one_scene = __ballot(path.scene != scene0) == 0ull;
+                // A packet walk visits the union of its lanes' nodes.  The first
+                // segments of a one-scene wave leave one light point for
+                // neighbouring first vertices and stay close; the later ones
+                // of a wave that holds many paths (one wavelength each) spread
+                // and walk per lane.
+                const uint32_t prev_pi = (uint32_t)__shfl_up((int)pi, 1, 64);
+                const uint32_t n_paths_wave =
+                    (uint32_t)__popcll(__ballot(in_range && (lane == 0u || prev_pi != pi)));
                 bool visible = 0.0f < thr;
 #pragma unroll
                 for (int i = 0; i < K + 1; ++i) {
                     const v3 ird = inv_rd(seg[i].rd);
                     bool hit;
+                    const bool packet = one_scene && (i == 0 || n_paths_wave <= kPacketPaths);
                     if (packet) {
                         const palz_scene_desc* us = scenes + scene0;
                         if (us->wide) {
